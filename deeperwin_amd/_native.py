@@ -1,0 +1,151 @@
+"""ctypes binding of ``libdwmi.so`` (the C ABI in ``include/dwmi.h``).
+
+The library must exist (built by ``python -m deeperwin_amd.build``); there is no CPU fallback, so a
+missing library raises immediately.  ``NativeModel`` owns one ``dw_model`` handle and moves torch
+tensors across the ABI as raw device pointers on torch's current HIP stream.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+from typing import Dict, Optional
+
+import numpy as np
+
+from .params import ModelShape
+
+LIB_PATH = Path(os.environ.get("DWMI_LIB", Path(__file__).resolve().parent / "libdwmi.so"))
+
+DW_FP32, DW_BF16X3, DW_BF16 = 0, 1, 2
+PRECISIONS = {"fp32": DW_FP32, "bf16x3": DW_BF16X3, "bf16": DW_BF16}
+FLAG_NONFINITE, FLAG_FP64_DET, FLAG_FORCED_ACCEPT = 1, 2, 4
+
+# Exported symbols and their ctypes signatures (argtypes, restype).
+_vp, _i32, _f32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_float
+
+
+class ModelDesc(ctypes.Structure):
+    _fields_ = [("n_el", ctypes.c_int32), ("n_up", ctypes.c_int32), ("n_ion", ctypes.c_int32),
+                ("n_det", ctypes.c_int32), ("width1", ctypes.c_int32), ("width2", ctypes.c_int32),
+                ("n_iter", ctypes.c_int32), ("precision", ctypes.c_int32), ("fp64_cond_threshold", ctypes.c_float)]
+
+
+SIGNATURES = {
+    "dw_model_create": ([ctypes.POINTER(ModelDesc), ctypes.POINTER(_vp)], ctypes.c_int),
+    "dw_model_destroy": ([_vp], ctypes.c_int),
+    "dw_model_set_param": ([_vp, ctypes.c_char_p, ctypes.c_char_p, _vp, ctypes.POINTER(ctypes.c_int64), _i32],
+                           ctypes.c_int),
+    "dw_model_ready": ([_vp], ctypes.c_int32),
+    "dw_logpsi": ([_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "dw_local_energy": ([_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "dw_mcmc_steps": ([_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _f32, _f32, _f32, _vp,
+                       _vp, _vp], ctypes.c_int),
+    "dw_workspace_size": ([_vp, _i32, _i32], ctypes.c_size_t),
+    "dw_model_set_chunk": ([_vp, _i32], ctypes.c_int),
+    "dw_last_error": ([], ctypes.c_char_p),
+    "dw_version": ([], ctypes.c_char_p),
+}
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def load_library() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m deeperwin_amd.build` "
+                           "(the native path has no CPU fallback)")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    for name, (argtypes, restype) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = restype
+    _lib = lib
+    return lib
+
+
+class DwError(RuntimeError):
+    pass
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        msg = load_library().dw_last_error().decode()
+        raise DwError(f"dwmi error {status}: {msg}")
+
+
+def _ptr(t) -> int:
+    return 0 if t is None else int(t.data_ptr())
+
+
+def _stream() -> int:
+    import torch
+
+    return int(torch.cuda.current_stream().cuda_stream)
+
+
+class NativeModel:
+    """One compiled-shape wavefunction on the current GPU."""
+
+    def __init__(self, shape: ModelShape, precision: str = "fp32", fp64_cond_threshold: float = 0.0,
+                 chunk: int = 0):
+        self.lib = load_library()
+        self.shape = shape
+        self.precision = precision
+        desc = ModelDesc(shape.n_el, shape.n_up, shape.n_ion, shape.n_det, shape.width1, shape.width2,
+                         shape.n_iter, PRECISIONS[precision], float(fp64_cond_threshold))
+        h = ctypes.c_void_p()
+        _check(self.lib.dw_model_create(ctypes.byref(desc), ctypes.byref(h)))
+        self.handle = h
+        self._param_token = None
+        if chunk:
+            _check(self.lib.dw_model_set_chunk(self.handle, int(chunk)))
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                self.lib.dw_model_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+    def set_params(self, params: Dict) -> None:
+        token = tuple(id(v) for leaves in params.values() for v in leaves.values())
+        if token == self._param_token:
+            return
+        for path, leaves in params.items():
+            for leaf, val in leaves.items():
+                arr = val.detach().cpu().numpy() if hasattr(val, "detach") else np.asarray(val)
+                arr = np.ascontiguousarray(arr, dtype=np.float32)
+                shp = (ctypes.c_int64 * arr.ndim)(*arr.shape)
+                _check(self.lib.dw_model_set_param(self.handle, path.encode(), leaf.encode(),
+                                                   arr.ctypes.data_as(ctypes.c_void_p), shp, arr.ndim))
+        if not self.lib.dw_model_ready(self.handle):
+            raise DwError("not every parameter leaf of the wavefunction was provided")
+        self._param_token = token
+
+    def set_chunk(self, walkers: int) -> None:
+        _check(self.lib.dw_model_set_chunk(self.handle, int(walkers)))
+
+    def workspace_bytes(self, n_walkers: int, mode: int) -> int:
+        return int(self.lib.dw_workspace_size(self.handle, int(n_walkers), int(mode)))
+
+    # ---- compute entry points (torch CUDA tensors, float32, contiguous) ----
+    def logpsi(self, r, R, Z, out_lp, out_phase=None) -> None:
+        _check(self.lib.dw_logpsi(self.handle, _ptr(r), r.shape[0], _ptr(R), _ptr(Z), _ptr(out_lp),
+                                  _ptr(out_phase), _stream()))
+
+    def local_energy(self, r, R, Z, out_eloc, out_lp=None, out_flags=None) -> None:
+        _check(self.lib.dw_local_energy(self.handle, _ptr(r), r.shape[0], _ptr(R), _ptr(Z), _ptr(out_eloc),
+                                        _ptr(out_lp), _ptr(out_flags), _stream()))
+
+    def mcmc_steps(self, r, R, Z, lp, age, normals, uniforms, chain, max_age, interval, target, lo, hi,
+                   out_acc=None, out_flags=None) -> None:
+        n_steps = 0 if normals is None else normals.shape[0]
+        _check(self.lib.dw_mcmc_steps(self.handle, _ptr(r), r.shape[0], _ptr(R), _ptr(Z), _ptr(lp), _ptr(age),
+                                      _ptr(normals), _ptr(uniforms), n_steps, _ptr(chain), int(max_age),
+                                      int(interval), float(target), float(lo), float(hi), _ptr(out_acc),
+                                      _ptr(out_flags), _stream()))

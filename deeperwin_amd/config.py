@@ -1,0 +1,160 @@
+"""Configuration records for the walker-parallel VMC inner loop.
+
+Only the fields the MCMC step, the log|psi| forward and the local energy read are kept.  Defaults follow
+the reference's pydantic defaults for the FermiNet-style model:
+
+* MLP: tanh, uniform fan_avg weights, zero bias        (src/deeperwin/configuration.py:215-227)
+* FermiNet embedding: 4 iterations, 256 / 32 widths    (configuration.py:277-375)
+* FermiNet input features                              (configuration.py:634-652)
+* envelope orbitals, full determinant, 32 dets         (configuration.py:688-699, 783-806)
+* MCMC: Gaussian proposal, target acceptance 0.5,
+  stepsize clip [1e-2, 1.0], evaluation max_age 100    (configuration.py:978-1050)
+* local energy: chunks of 64 walkers, forward_lap off  (configuration.py:1264-1266)
+"""
+from __future__ import annotations
+
+import dataclasses
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+
+@dataclass
+class MLPConfig:
+    activation: str = "tanh"
+    init_bias_scale: float = 0.0
+    init_weights_scale: str = "fan_avg"
+    init_weights_distribution: str = "uniform"
+    use_residual: bool = False
+    use_layer_norm: bool = False
+
+
+@dataclass
+class EmbeddingConfigFermiNet:
+    name: str = "ferminet"
+    n_hidden_one_el: int = 256
+    n_hidden_two_el: int = 32
+    n_iterations: int = 4
+    use_average_h_one: bool = True
+    use_average_h_two: bool = True
+    use_h_one: bool = True
+    use_h_two_same_diff: bool = False
+    use_el_ion_stream: bool = False
+    use_schnet_features: bool = False
+
+
+@dataclass
+class EnvelopeOrbitalsConfig:
+    envelope_type: str = "isotropic_exp"
+    n_hidden: List[int] = field(default_factory=list)
+    use_bias: bool = False
+    initialization: str = "constant"
+
+
+@dataclass
+class OrbitalsConfigFermiNet:
+    envelope_orbitals: EnvelopeOrbitalsConfig = field(default_factory=EnvelopeOrbitalsConfig)
+    n_determinants: int = 32
+    determinant_schema: str = "full_det"
+
+
+@dataclass
+class ModelConfigFermiNet:
+    name: str = "ferminet"
+    embedding: EmbeddingConfigFermiNet = field(default_factory=EmbeddingConfigFermiNet)
+    orbitals: OrbitalsConfigFermiNet = field(default_factory=OrbitalsConfigFermiNet)
+    mlp: MLPConfig = field(default_factory=MLPConfig)
+    jastrow: Optional[object] = None
+    use_el_el_cusp_correction: bool = False
+    complex_wf: bool = False
+
+    def validate(self) -> None:
+        """Reject settings the native path does not implement (raises, as the reference does on
+        unknown options, e.g. envelope_orbitals.py:81,85)."""
+        e, o, m = self.embedding, self.orbitals, self.mlp
+        if e.name != "ferminet":
+            raise NotImplementedError(f"embedding {e.name!r} is not on the native path")
+        if m.activation != "tanh" or m.use_layer_norm or m.use_residual:
+            raise NotImplementedError("native MLP path is tanh, no layer norm, no global residual")
+        if e.use_h_two_same_diff or e.use_el_ion_stream or e.use_schnet_features:
+            raise NotImplementedError("only the default FermiNet symmetric features are implemented")
+        if not (e.use_average_h_one and e.use_average_h_two and e.use_h_one):
+            raise NotImplementedError("only the default FermiNet symmetric features are implemented")
+        if o.determinant_schema != "full_det":
+            raise NotImplementedError("only determinant_schema='full_det' is implemented")
+        if o.envelope_orbitals.envelope_type != "isotropic_exp" or o.envelope_orbitals.n_hidden:
+            raise NotImplementedError("only isotropic_exp envelopes without hidden backflow layers")
+        if o.envelope_orbitals.use_bias:
+            raise NotImplementedError("backflow output bias is off in the FermiNet model")
+        if self.jastrow or self.use_el_el_cusp_correction or self.complex_wf:
+            raise NotImplementedError("jastrow / cusp / complex wavefunctions are off in the FermiNet model")
+
+
+@dataclass
+class MCMCSimpleProposalConfig:
+    name: str = "normal"
+
+
+@dataclass
+class MCMCConfig:
+    """Evaluation defaults (configuration.py:1045-1049); the configs fix the Gaussian init and
+    stepsize 0.02 bohr instead of the reference's exponential init and 1e-2."""
+    n_inter_steps: int = 20
+    n_burn_in: int = 500
+    max_age: int = 100
+    stepsize_update_interval: int = 100
+    n_walkers: int = 2048
+    initialization: str = "gaussian"
+    spin_initialization: str = "el_ion_mapping"
+    target_acceptance_rate: float = 0.5
+    min_stepsize_scale: float = 1e-2
+    max_stepsize_scale: float = 1.0
+    proposal: MCMCSimpleProposalConfig = field(default_factory=MCMCSimpleProposalConfig)
+    p_spin_swap: float = 0.0
+    p_spin_flip: float = 0.0
+    stepsize: float = 0.02
+
+    def validate(self) -> None:
+        if self.proposal.name != "normal":
+            raise NotImplementedError(f"proposal {self.proposal.name!r} is not on the native path")
+        if self.p_spin_swap > 0 or self.p_spin_flip > 0:
+            raise NotImplementedError("spin swap / spin flip moves are not on the native path")
+
+
+@dataclass
+class PhysicalConfig:
+    """Molecule description (configuration.py:1537-1745): R in bohr, integer charges Z, electron counts."""
+    R: List[List[float]]
+    Z: List[int]
+    n_electrons: Optional[int] = None
+    n_up: Optional[int] = None
+    el_ion_mapping: Optional[List[int]] = None
+    name: Optional[str] = None
+    E_ref: Optional[float] = None
+
+    def __post_init__(self):
+        from .physics import generate_el_ion_mapping
+
+        if self.n_electrons is None:
+            self.n_electrons = int(sum(self.Z))
+        if self.n_up is None:
+            # configuration.py:1740 with spin 0: extra electron goes to spin up
+            self.n_up = (self.n_electrons + 1) // 2
+        if self.el_ion_mapping is None:
+            self.el_ion_mapping = generate_el_ion_mapping(self.R, self.Z, self.n_electrons, self.n_up)
+
+    @property
+    def n_dn(self) -> int:
+        return self.n_electrons - self.n_up
+
+    @property
+    def n_ions(self) -> int:
+        return len(self.Z)
+
+    def get_basic_params(self):
+        import numpy as np
+
+        return self.n_electrons, self.n_up, np.array(self.R, dtype=np.float64), np.array(self.Z)
+
+
+def replace(obj, **kw):
+    return dataclasses.replace(obj, **kw)

@@ -1,0 +1,735 @@
+// Streaming, per-walker and per-determinant kernels of the FermiNet-style wavefunction on gfx950.
+//
+// Activations are stored one row per (walker b, electron i, channel ch) with C channels per electron:
+//   C == 1        : value only (log|psi| / MCMC)
+//   C == 3 n + 2  : ch 0 value, ch 1 + 3 e + c the tangent along coordinate c of electron e,
+//                   ch 3 n + 1 the Laplacian (sum over all 3 n second derivatives)
+// which is the forward-mode Laplacian of the local energy.  Rows of a 1-el layer input X_l hold
+//   [ h_l (Kh) | mean over spin-up partners of the 2-el stream (Ke) | same over spin-down (Ke) | 0 pad ]
+// The spin means of h (the per-walker part of the symmetric features) are applied through a separate
+// per-(walker, channel) GEMM, so the 1-el GEMM contracts over Kh + 2 Ke instead of 3 Kh + 2 Ke.
+#include "common.h"
+#include "kernels.h"
+
+namespace dwmi {
+
+static constexpr int MAXL = 4;  // FermiNet iterations supported by the fused 2-el kernel
+
+// ------------------------------------------------------------------------------------------------
+// Layer-0 one-electron features: per ion [|r_i - R_I|, r_i - R_I] (utils.py:262-273,
+// input_features.py:41-68, 237-239), with their tangents (only along electron i) and Laplacians
+// (Laplacian of |x| in 3D is 2/|x|; of a coordinate difference 0).
+// ------------------------------------------------------------------------------------------------
+__global__ void k_features0(const float* __restrict__ r, const float* __restrict__ R, float* __restrict__ X0,
+                            int Kx0, Shape sh) {
+  const long total = (long)sh.B * sh.n * sh.C * Kx0;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int col = (int)(t % Kx0);
+  const long row = t / Kx0;
+  const int ch = (int)(row % sh.C);
+  const long bi = row / sh.C;
+  const int i = (int)(bi % sh.n);
+  const int I4 = 4 * sh.I;
+  if (col >= I4 && col < I4 + 8) return;  // pair-mean columns: written by k_twoel
+  float v = 0.f;
+  if (col < I4) {
+    const int ion = col >> 2, f = col & 3;
+    const float* ri = r + bi * 3;
+    const float d[3] = {ri[0] - R[ion * 3 + 0], ri[1] - R[ion * 3 + 1], ri[2] - R[ion * 3 + 2]};
+    const float rho = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (ch == 0) {
+      v = (f == 0) ? rho : d[f - 1];
+    } else if (ch == sh.C - 1) {
+      v = (f == 0) ? 2.f / rho : 0.f;
+    } else {
+      const int k = ch - 1, e = k / 3, c = k % 3;
+      if (e == i) v = (f == 0) ? d[c] / rho : ((f - 1 == c) ? 1.f : 0.f);
+    }
+  }
+  X0[t] = v;
+}
+
+void launch_features0(const float* r, const float* R, float* X0, int Kx0, const Shape& sh, hipStream_t s) {
+  const long total = (long)sh.B * sh.n * sh.C * Kx0;
+  hipLaunchKernelGGL(k_features0, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, r, R, X0, Kx0, sh);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Fused two-electron stream.  One workgroup per (walker b, electron i); 256 threads = 8 partner slots
+// x 32 features.  For each partner j the pair features [|r_j - r_i|, r_j - r_i] (utils.py:287-305) go
+// through the 2-el MLPs 'h_el_el_l' (tanh, residual (x+y)/sqrt(2) when widths agree; mlp.py:13-16,
+// 45-68) carrying the value, the 3 tangents along d = r_j - r_i and the Laplacian.  Because a pair
+// feature depends on r_j - r_i only, d/dr_j = +T and d/dr_i = -T, and the Laplacian over all 3 n
+// directions sees each |T_c|^2 twice.  The spin means over partners (ferminet_embedding.py:79-81;
+// the self pair is included) are written into the pair columns of every layer input X_l:
+//   value / Laplacian rows : mean over j in the spin block
+//   tangent row (e, c)     : T_c[i, e] / n_s(e) in block s(e)  (e != i);  -sum_j T_c[i, j] / n_s (e == i)
+// Partial sums are kept per slot and reduced in a fixed order, so results are deterministic.
+// ------------------------------------------------------------------------------------------------
+struct TwoElOut {
+  float* X[MAXL];
+  int Kx[MAXL];
+  int Kh[MAXL];
+};
+
+template <bool LAP>
+__global__ __launch_bounds__(256) void k_twoel(const float* __restrict__ r, TwoElOut out,
+                                               const float* __restrict__ W2e, const float* __restrict__ b2e,
+                                               Shape sh) {
+  const int L = sh.L, n = sh.n, u = sh.u, C = sh.C;
+  const long bi = blockIdx.x;
+  const int i = (int)(bi % n);
+  const long b = bi / n;
+  const int tid = threadIdx.x, slot = tid >> 5, f = tid & 31;
+  constexpr int NCH = LAP ? 5 : 1;
+  __shared__ float Ws[MAXL - 1][32][32];
+  __shared__ float bs[MAXL - 1][32];
+  __shared__ float xin[8][5][32];
+  __shared__ float part[8][MAXL][2][5][32];
+
+  for (int idx = tid; idx < (L - 1) * 32 * 32; idx += 256) (&Ws[0][0][0])[idx] = W2e[idx];
+  for (int idx = tid; idx < (L - 1) * 32; idx += 256) (&bs[0][0])[idx] = b2e[idx];
+  for (int idx = tid; idx < L * 2 * 5 * 32; idx += 256) {
+    const int ff = idx & 31, rest = idx >> 5;  // rest = (l * 2 + s) * 5 + ch
+    const int ch = rest % 5, ls = rest / 5;
+#pragma unroll
+    for (int sl = 0; sl < 8; ++sl) part[sl][ls >> 1][ls & 1][ch][ff] = 0.f;
+  }
+  const float* rb = r + b * n * 3;
+  const float rix = rb[i * 3 + 0], riy = rb[i * 3 + 1], riz = rb[i * 3 + 2];
+  const float inv_up = 1.f / (float)u, inv_dn = 1.f / (float)(n - u);
+  const float rs2 = 0.70710678118654752440f;
+  __syncthreads();
+
+  for (int j0 = 0; j0 < n; j0 += 8) {
+    const int j = j0 + slot;
+    const bool valid = j < n;
+    const int sj = (j < u) ? 0 : 1;
+    const float inv_s = sj == 0 ? inv_up : inv_dn;
+    // ---- raw pair features (layer-0 2-el input) ----
+    float xv = 0.f, xt0 = 0.f, xt1 = 0.f, xt2 = 0.f, xl = 0.f;
+    if (valid && j != i && f < 4) {
+      const float dx = rb[j * 3 + 0] - rix, dy = rb[j * 3 + 1] - riy, dz = rb[j * 3 + 2] - riz;
+      const float rho = sqrtf(dx * dx + dy * dy + dz * dz);
+      if (f == 0) {
+        xv = rho;
+        xt0 = dx / rho;
+        xt1 = dy / rho;
+        xt2 = dz / rho;
+        xl = 4.f / rho;
+      } else {
+        xv = (f == 1) ? dx : (f == 2 ? dy : dz);
+        xt0 = (f == 1) ? 1.f : 0.f;
+        xt1 = (f == 2) ? 1.f : 0.f;
+        xt2 = (f == 3) ? 1.f : 0.f;
+      }
+    }
+    if (valid && f < 4) {
+      part[slot][0][sj][0][f] += xv;
+      if (LAP) {
+        part[slot][0][sj][1][f] += xt0;
+        part[slot][0][sj][2][f] += xt1;
+        part[slot][0][sj][3][f] += xt2;
+        part[slot][0][sj][4][f] += xl;
+        if (j != i) {
+          const float tv[3] = {xt0, xt1, xt2};
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            float* row = out.X[0] + ((bi * C) + 1 + 3 * j + c) * (long)out.Kx[0] + out.Kh[0];
+            row[sj * 4 + f] = tv[c] * inv_s;
+            row[(1 - sj) * 4 + f] = 0.f;
+          }
+        }
+      }
+    }
+    xin[slot][0][f] = xv;
+    if (LAP) {
+      xin[slot][1][f] = xt0;
+      xin[slot][2][f] = xt1;
+      xin[slot][3][f] = xt2;
+      xin[slot][4][f] = xl;
+    }
+    __syncthreads();
+    // ---- 2-el MLP layers ----
+    for (int l2 = 0; l2 < L - 1; ++l2) {
+      const int kin = (l2 == 0) ? 4 : 32;
+      float zv = bs[l2][f], z0 = 0.f, z1 = 0.f, z2 = 0.f, zl = 0.f;
+      for (int k = 0; k < kin; ++k) {
+        const float w = Ws[l2][k][f];
+        zv += xin[slot][0][k] * w;
+        if (LAP) {
+          z0 += xin[slot][1][k] * w;
+          z1 += xin[slot][2][k] * w;
+          z2 += xin[slot][3][k] * w;
+          zl += xin[slot][4][k] * w;
+        }
+      }
+      const float th = tanhf(zv);
+      const float d1 = 1.f - th * th, d2 = -2.f * th * d1;
+      float y[5];
+      y[0] = th;
+      if (LAP) {
+        y[1] = d1 * z0;
+        y[2] = d1 * z1;
+        y[3] = d1 * z2;
+        y[4] = d1 * zl + d2 * 2.f * (z0 * z0 + z1 * z1 + z2 * z2);
+      }
+      if (kin == 32) {  // residual: input and output widths agree
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) y[c] = (xin[slot][c][f] + y[c]) * rs2;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) xin[slot][c][f] = y[c];
+      if (valid) {
+        const int l = l2 + 1;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) part[slot][l][sj][c][f] += y[c];
+        if (LAP && j != i) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            float* row = out.X[l] + ((bi * C) + 1 + 3 * j + c) * (long)out.Kx[l] + out.Kh[l];
+            row[sj * 32 + f] = y[1 + c] * inv_s;
+            row[(1 - sj) * 32 + f] = 0.f;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  // ---- reduce the 8 slots in a fixed order and write value / Laplacian / self-tangent rows ----
+  for (int l = 0; l < L; ++l) {
+    const int Ke = (l == 0) ? 4 : 32;
+    for (int idx = tid; idx < 2 * NCH * Ke; idx += 256) {
+      const int ff = idx % Ke, rest = idx / Ke;
+      const int c = rest % NCH, s = rest / NCH;
+      float acc = 0.f;
+#pragma unroll
+      for (int sl = 0; sl < 8; ++sl) acc += part[sl][l][s][c][ff];
+      const float inv = (s == 0) ? inv_up : inv_dn;
+      long chrow;
+      float v;
+      if (c == 0) {
+        chrow = 0;
+        v = acc * inv;
+      } else if (c == 4) {
+        chrow = C - 1;
+        v = acc * inv;
+      } else {
+        chrow = 1 + 3 * i + (c - 1);
+        v = -acc * inv;
+      }
+      out.X[l][((bi * C) + chrow) * (long)out.Kx[l] + out.Kh[l] + s * Ke + ff] = v;
+    }
+  }
+}
+
+void launch_twoel(const float* r, float* const* X, const int* Kx, const int* Kh, const float* W2e, const float* b2e,
+                  const Shape& sh, hipStream_t s) {
+  TwoElOut o;
+  for (int l = 0; l < MAXL; ++l) {
+    o.X[l] = l < sh.L ? X[l] : nullptr;
+    o.Kx[l] = l < sh.L ? Kx[l] : 0;
+    o.Kh[l] = l < sh.L ? Kh[l] : 0;
+  }
+  const unsigned grid = (unsigned)(sh.B * sh.n);
+  if (sh.C > 1)
+    hipLaunchKernelGGL(k_twoel<true>, dim3(grid), dim3(256), 0, s, r, o, W2e, b2e, sh);
+  else
+    hipLaunchKernelGGL(k_twoel<false>, dim3(grid), dim3(256), 0, s, r, o, W2e, b2e, sh);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Spin means of h over electrons (ferminet_embedding.py:61-62, 72): MU[b, ch] = [mean_{i<u} h, mean_{i>=u} h].
+// ------------------------------------------------------------------------------------------------
+__global__ void k_spin_mean(const float* __restrict__ X, int Kx, int Kh, float* __restrict__ MU, int Km, Shape sh) {
+  const long total = (long)sh.B * sh.C * Km;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int f = (int)(t % Km);
+  const long row = t / Km;
+  const int ch = (int)(row % sh.C);
+  const long b = row / sh.C;
+  float v = 0.f;
+  if (f < 2 * Kh) {
+    const int s = f >= Kh ? 1 : 0;
+    const int fi = f - s * Kh;
+    const int i0 = s ? sh.u : 0, i1 = s ? sh.n : sh.u;
+    float acc = 0.f;
+    for (int i = i0; i < i1; ++i) acc += X[((b * sh.n + i) * sh.C + ch) * (long)Kx + fi];
+    v = acc / (float)(i1 - i0);
+  }
+  MU[t] = v;
+}
+
+void launch_spin_mean(const float* X, int Kx, int Kh, float* MU, int Km, const Shape& sh, hipStream_t s) {
+  const long total = (long)sh.B * sh.C * Km;
+  hipLaunchKernelGGL(k_spin_mean, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, X, Kx, Kh, MU, Km, sh);
+}
+
+// ------------------------------------------------------------------------------------------------
+// tanh with forward-mode Laplacian: y = tanh(z); dy_k = t' dz_k; Lap y = t' Lap z + t'' sum_k dz_k^2.
+// One thread per (walker, electron, feature); channels are read from the pre-activation rows.
+// ------------------------------------------------------------------------------------------------
+__global__ void k_tanh_rule(const float* __restrict__ Z, float* __restrict__ out, int ldo, int permute, Shape sh) {
+  const long total = (long)sh.B * sh.n * sh.W1;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int f = (int)(t % sh.W1);
+  const long bi = t / sh.W1;
+  const int i = (int)(bi % sh.n);
+  const long b = bi / sh.n;
+  const int C = sh.C;
+  const float* z = Z + bi * C * (long)sh.W1 + f;
+  long obase;
+  if (permute) {
+    const int dn = sh.n - sh.u;
+    obase = (i < sh.u) ? (b * sh.u + i) * C : (long)sh.B * sh.u * C + (b * dn + (i - sh.u)) * C;
+  } else {
+    obase = bi * C;
+  }
+  const float th = tanhf(z[0]);
+  const float d1 = 1.f - th * th, d2 = -2.f * th * d1;
+  out[obase * ldo + f] = th;
+  float S = 0.f;
+  for (int ch = 1; ch < C - 1; ++ch) {
+    const float zc = z[(long)ch * sh.W1];
+    S += zc * zc;
+    out[(obase + ch) * ldo + f] = d1 * zc;
+  }
+  out[(obase + C - 1) * ldo + f] = d1 * z[(long)(C - 1) * sh.W1] + d2 * S;
+}
+
+void launch_tanh_rule(const float* Z, float* out, int ldo, int permute, const Shape& sh, hipStream_t s) {
+  const long total = (long)sh.B * sh.n * sh.W1;
+  hipLaunchKernelGGL(k_tanh_rule, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, Z, out, ldo, permute, sh);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Envelope x backflow (envelope_orbitals.py:91-127): A[b, d, i, k] = env_s[i, c] * bf[i, c], c = d n + k,
+// env_s[i, c] = sum_I w_s[I, c] exp(-kappa_s[I, c] |r_i - R_I|), kappa = softplus(alpha), with the product
+// rule for tangents (env depends on electron i only) and the Laplacian
+//   Lap A = Lap(env) bf + 2 grad_i(env) . grad_i(bf) + env Lap(bf),   Lap e^{-k rho} = (k^2 - 2k/rho) e^{-k rho}.
+// O rows are spin-major (up block then down block); A is [B, D, C, n, n].
+// ------------------------------------------------------------------------------------------------
+__global__ void k_envelope(const float* __restrict__ O, int ldo, const float* __restrict__ r,
+                           const float* __restrict__ R, const float* __restrict__ kappa,
+                           const float* __restrict__ wenv, float* __restrict__ A, Shape sh) {
+  const int n = sh.n, NO = sh.D * n, C = sh.C;
+  const long total = (long)sh.B * n * NO;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int col = (int)(t % NO);
+  const long bi = t / NO;
+  const int i = (int)(bi % n);
+  const long b = bi / n;
+  const int s = (i < sh.u) ? 0 : 1;
+  const int dn = n - sh.u;
+  const long orow = (s == 0) ? (b * sh.u + i) * C : (long)sh.B * sh.u * C + (b * dn + (i - sh.u)) * C;
+  const float* ri = r + bi * 3;
+  const float* kap = kappa + (long)s * sh.I * NO;
+  const float* we = wenv + (long)s * sh.I * NO;
+  float env = 0.f, g0 = 0.f, g1 = 0.f, g2 = 0.f, lenv = 0.f;
+  for (int I = 0; I < sh.I; ++I) {
+    const float dx = ri[0] - R[I * 3 + 0], dy = ri[1] - R[I * 3 + 1], dz = ri[2] - R[I * 3 + 2];
+    const float rho = sqrtf(dx * dx + dy * dy + dz * dz);
+    const float k = kap[I * NO + col];
+    const float e = we[I * NO + col] * expf(-k * rho);
+    env += e;
+    if (C > 1) {
+      const float coef = -k * e / rho;
+      g0 += coef * dx;
+      g1 += coef * dy;
+      g2 += coef * dz;
+      lenv += e * (k * k - 2.f * k / rho);
+    }
+  }
+  const int d = col / n, k = col % n;
+  const long nn = (long)n * n;
+  float* Ab = A + ((b * sh.D + d) * C) * nn + (long)i * n + k;
+  const float bf0 = O[orow * ldo + col];
+  Ab[0] = env * bf0;
+  if (C > 1) {
+    for (int ch = 1; ch < C - 1; ++ch) {
+      const int e = (ch - 1) / 3, c = (ch - 1) % 3;
+      float v = env * O[(orow + ch) * ldo + col];
+      if (e == i) v += (c == 0 ? g0 : (c == 1 ? g1 : g2)) * bf0;
+      Ab[(long)ch * nn] = v;
+    }
+    const float t0 = O[(orow + 1 + 3 * i) * ldo + col];
+    const float t1 = O[(orow + 2 + 3 * i) * ldo + col];
+    const float t2 = O[(orow + 3 + 3 * i) * ldo + col];
+    Ab[(long)(C - 1) * nn] = lenv * bf0 + env * O[(orow + C - 1) * ldo + col] + 2.f * (g0 * t0 + g1 * t1 + g2 * t2);
+  }
+}
+
+void launch_envelope(const float* O, int ldo, const float* r, const float* R, const float* kappa, const float* wenv,
+                     float* A, const Shape& sh, hipStream_t s) {
+  const long total = (long)sh.B * sh.n * sh.D * sh.n;
+  hipLaunchKernelGGL(k_envelope, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, O, ldo, r, R, kappa, wenv,
+                     A, sh);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Determinants: one workgroup per (walker, det).  Partial-pivot LU in LDS (first maximal |pivot| wins,
+// as LAPACK getrf), sign and log|det| (jnp.linalg.slogdet, wavefunction.py:69).  In Laplacian mode the
+// inverse is formed and for every tangent direction k: g_k = tr(M^-1 dM_k), q_k = tr((M^-1 dM_k)^2);
+// the Laplacian of log|det| is tr(M^-1 Lap M) - sum_k q_k.  A condition proxy max|U_pp| / min|U_pp| is
+// reported through dflags so that flagged determinants can be refactored in FP64.
+// ------------------------------------------------------------------------------------------------
+template <bool LAP>
+__global__ __launch_bounds__(256) void k_det(const float* __restrict__ A, float* __restrict__ ld,
+                                             float* __restrict__ sd, float* __restrict__ g, float* __restrict__ lapd,
+                                             uint32_t* __restrict__ dflags, float cond_thr, Shape sh) {
+  extern __shared__ float smem[];
+  const int n = sh.n, P = n + 1, C = sh.C;
+  float* M = smem;
+  float* Mi = M + n * P;
+  float* T = Mi + n * P;
+  float* lcol = T + n * P;
+  float* red = lcol + 64;
+  int* perm = (int*)(red + 32);
+  __shared__ int piv_sh;
+  __shared__ int sing_sh;
+  const long bd = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const long nn = (long)n * n;
+  const float* Ab = A + bd * C * nn;
+  for (int e = tid; e < n * n; e += 256) M[(e / n) * P + (e % n)] = Ab[e];
+  if (tid < n) perm[tid] = tid;
+  if (tid == 0) sing_sh = 0;
+  __syncthreads();
+  int swaps = 0;
+  for (int p = 0; p < n; ++p) {
+    if (wid == 0) {
+      float best = -1.f;
+      int bidx = p;
+      const int rr = p + lane;
+      if (rr < n) {
+        best = fabsf(M[rr * P + p]);
+        bidx = rr;
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const float ob = __shfl_xor(best, o, 64);
+        const int oi = __shfl_xor(bidx, o, 64);
+        if (ob > best || (ob == best && oi < bidx)) {
+          best = ob;
+          bidx = oi;
+        }
+      }
+      if (lane == 0) {
+        piv_sh = bidx;
+        if (!(best > 0.f)) sing_sh = 1;
+      }
+    }
+    __syncthreads();
+    const int piv = piv_sh;
+    if (piv != p) {
+      for (int j = tid; j < n; j += 256) {
+        const float tmp = M[p * P + j];
+        M[p * P + j] = M[piv * P + j];
+        M[piv * P + j] = tmp;
+      }
+      if (tid == 0) {
+        const int tp = perm[p];
+        perm[p] = perm[piv];
+        perm[piv] = tp;
+      }
+      ++swaps;
+    }
+    __syncthreads();
+    const float pivot = M[p * P + p];
+    for (int rr = p + 1 + tid; rr < n; rr += 256) {
+      const float l = M[rr * P + p] / pivot;
+      lcol[rr] = l;
+      M[rr * P + p] = l;
+    }
+    __syncthreads();
+    const int m = n - p - 1;
+    for (int e = tid; e < m * m; e += 256) {
+      const int rr = p + 1 + e / m, cc = p + 1 + e % m;
+      M[rr * P + cc] -= lcol[rr] * M[p * P + cc];
+    }
+    __syncthreads();
+  }
+  // sign / log|det| / condition proxy
+  if (tid == 0) {
+    float sgn = (swaps & 1) ? -1.f : 1.f;
+    float lsum = 0.f, umax = 0.f, umin = 3.4e38f;
+    for (int p = 0; p < n; ++p) {
+      const float up = M[p * P + p];
+      if (up < 0.f) sgn = -sgn;
+      const float a = fabsf(up);
+      lsum += logf(a);
+      umax = fmaxf(umax, a);
+      umin = fminf(umin, a);
+    }
+    if (sing_sh) {
+      sgn = 0.f;
+      lsum = -INFINITY;
+    }
+    ld[bd] = lsum;
+    sd[bd] = sgn;
+    uint32_t fl = 0;
+    if (cond_thr > 0.f && !(umax <= cond_thr * umin)) fl |= 1u;
+    dflags[bd] = fl;
+  }
+  if (!LAP) return;
+  // inverse: column j solves L U x = P e_j
+  for (int j = tid; j < n; j += 256) {
+    for (int p = 0; p < n; ++p) {
+      float y = (perm[p] == j) ? 1.f : 0.f;
+      for (int q = 0; q < p; ++q) y -= M[p * P + q] * Mi[q * P + j];
+      Mi[p * P + j] = y;
+    }
+    for (int p = n - 1; p >= 0; --p) {
+      float x = Mi[p * P + j];
+      for (int q = p + 1; q < n; ++q) x -= M[p * P + q] * Mi[q * P + j];
+      Mi[p * P + j] = x / M[p * P + p];
+    }
+  }
+  __syncthreads();
+  const int nd = 3 * n;
+  float qacc = 0.f;
+  for (int ch = 1; ch <= nd; ++ch) {
+    const float* Tg = Ab + (long)ch * nn;
+    for (int e = tid; e < n * n; e += 256) T[(e / n) * P + (e % n)] = Tg[e];
+    __syncthreads();
+    // X = Mi * T  (reuse M)
+    for (int e = tid; e < n * n; e += 256) {
+      const int a = e / n, c = e % n;
+      float acc = 0.f;
+      for (int q = 0; q < n; ++q) acc += Mi[a * P + q] * T[q * P + c];
+      M[a * P + c] = acc;
+    }
+    __syncthreads();
+    float pd = 0.f, pq = 0.f;
+    for (int e = tid; e < n * n; e += 256) {
+      const int a = e / n, c = e % n;
+      const float x = M[a * P + c];
+      pq += x * M[c * P + a];
+      if (a == c) pd += x;
+    }
+    const float gs = block_sum(pd, red);
+    const float qs = block_sum(pq, red + 16);
+    if (tid == 0) g[bd * nd + (ch - 1)] = gs;
+    qacc += qs;
+    __syncthreads();
+  }
+  {
+    const float* Tl = Ab + (long)(C - 1) * nn;
+    float pt = 0.f;
+    for (int e = tid; e < n * n; e += 256) {
+      const int a = e / n, q = e % n;
+      pt += Mi[a * P + q] * Tl[q * n + a];
+    }
+    const float tr = block_sum(pt, red);
+    if (tid == 0) lapd[bd] = tr - qacc;
+  }
+}
+
+void launch_det(const float* A, float* ld, float* sd, float* g, float* lapd, uint32_t* dflags, float cond_thr,
+                const Shape& sh, hipStream_t s) {
+  const int n = sh.n;
+  const size_t shm = (size_t)(3 * n * (n + 1) + 64 + 32 + 64) * sizeof(float);
+  const unsigned grid = (unsigned)(sh.B * sh.D);
+  if (sh.C > 1)
+    hipLaunchKernelGGL(k_det<true>, dim3(grid), dim3(256), shm, s, A, ld, sd, g, lapd, dflags, cond_thr, sh);
+  else
+    hipLaunchKernelGGL(k_det<false>, dim3(grid), dim3(256), shm, s, A, ld, sd, g, lapd, dflags, cond_thr, sh);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Per-walker combination (FP64).  Sum of determinants with max shift and LOG_EPSILON
+// (wavefunction.py:63-83): f = log(|S| + 1e-8) + m, S = sum_d s_d exp(l_d - m), log_psi_sqr = 2 f.
+// Derivatives follow the same expression literally (the max routes its derivative to the arg-max set,
+// averaged over ties, like jnp.max):
+//   df_k   = sgn(S) dS_k / (|S| + eps) + dm_k,          dS_k = sum_d e_d (g_dk - dm_k)
+//   Lap f  = sgn(S) sum_k d2S_k / (|S| + eps) - sum_k dS_k^2 / (|S| + eps)^2 + Lap m
+//   sum_k d2S_k = sum_d e_d [ sum_k (g_dk - dm_k)^2 + Lap l_d - Lap m ]
+// E_kin = -1/2 (Lap f + |grad f|^2)  (hamiltonian.py:216, 266 with log psi^2 = 2 f); E_pot as
+// hamiltonian.py:17-39.
+// ------------------------------------------------------------------------------------------------
+template <bool LAP>
+__global__ void k_combine(const float* __restrict__ ld, const float* __restrict__ sd, const float* __restrict__ g,
+                          const float* __restrict__ lapd, const uint32_t* __restrict__ dflags,
+                          const float* __restrict__ r, const float* __restrict__ R, const float* __restrict__ Z,
+                          float* __restrict__ out_lp, float* __restrict__ out_phase, float* __restrict__ out_eloc,
+                          uint32_t* __restrict__ out_flags, Shape sh) {
+  const long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= sh.B) return;
+  const int D = sh.D, n = sh.n, nd = 3 * n;
+  const float* l = ld + b * D;
+  const float* sg = sd + b * D;
+  double m = -INFINITY;
+  for (int d = 0; d < D; ++d) m = fmax(m, (double)l[d]);
+  double S = 0.0;
+  int cnt = 0;
+  uint32_t fl = 0;
+  for (int d = 0; d < D; ++d) {
+    S += (double)sg[d] * exp((double)l[d] - m);
+    if ((double)l[d] == m) ++cnt;
+    if (dflags[b * D + d]) fl |= DWF_FP64;
+  }
+  const double eps = 1e-8;
+  const double aS = fabs(S);
+  const double fval = log(aS + eps) + m;
+  const float lp = (float)(2.0 * fval);
+  if (out_lp) out_lp[b] = lp;
+  if (out_phase) out_phase[b] = (S < 0.0) ? 3.14159265358979323846f : 0.f;
+  if (!isfinite(lp)) fl |= DWF_NONFINITE;
+  if (LAP) {
+    const double sgnS = (S > 0.0) ? 1.0 : ((S < 0.0) ? -1.0 : 0.0);
+    const double den = aS + eps;
+    const float* gb = g + b * D * nd;
+    double lapm = 0.0;
+    for (int d = 0; d < D; ++d)
+      if ((double)l[d] == m) lapm += lapd[b * D + d];
+    lapm /= cnt;
+    double sum_d2S = 0.0, sum_dS2 = 0.0, sum_df2 = 0.0;
+    for (int k = 0; k < nd; ++k) {
+      double dm = 0.0;
+      for (int d = 0; d < D; ++d)
+        if ((double)l[d] == m) dm += gb[d * nd + k];
+      dm /= cnt;
+      double dS = 0.0, d2 = 0.0;
+      for (int d = 0; d < D; ++d) {
+        const double e = (double)sg[d] * exp((double)l[d] - m);
+        const double x = (double)gb[d * nd + k] - dm;
+        dS += e * x;
+        d2 += e * x * x;
+      }
+      sum_d2S += d2;
+      sum_dS2 += dS * dS;
+      const double df = sgnS * dS / den + dm;
+      sum_df2 += df * df;
+    }
+    for (int d = 0; d < D; ++d) {
+      const double e = (double)sg[d] * exp((double)l[d] - m);
+      sum_d2S += e * ((double)lapd[b * D + d] - lapm);
+    }
+    const double lapf = sgnS * sum_d2S / den - sum_dS2 / (den * den) + lapm;
+    const double ekin = -0.5 * (lapf + sum_df2);
+    // potential energy
+    const float* rb = r + b * n * 3;
+    double epot = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = i + 1; j < n; ++j) {
+        const double dx = rb[i * 3] - rb[j * 3], dy = rb[i * 3 + 1] - rb[j * 3 + 1], dz = rb[i * 3 + 2] - rb[j * 3 + 2];
+        epot += 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+      }
+    for (int i = 0; i < n; ++i)
+      for (int I = 0; I < sh.I; ++I) {
+        const double dx = rb[i * 3] - R[I * 3], dy = rb[i * 3 + 1] - R[I * 3 + 1], dz = rb[i * 3 + 2] - R[I * 3 + 2];
+        epot -= (double)Z[I] / sqrt(dx * dx + dy * dy + dz * dz);
+      }
+    for (int I = 0; I < sh.I; ++I)
+      for (int J = I + 1; J < sh.I; ++J) {
+        const double dx = R[I * 3] - R[J * 3], dy = R[I * 3 + 1] - R[J * 3 + 1], dz = R[I * 3 + 2] - R[J * 3 + 2];
+        epot += (double)Z[I] * Z[J] / sqrt(dx * dx + dy * dy + dz * dz);
+      }
+    const float eloc = (float)(ekin + epot);
+    if (!isfinite(eloc)) fl |= DWF_NONFINITE;
+    if (out_eloc) out_eloc[b] = eloc;
+  }
+  if (out_flags) out_flags[b] = fl;
+}
+
+void launch_combine(const float* ld, const float* sd, const float* g, const float* lapd, const uint32_t* dflags,
+                    const float* r, const float* R, const float* Z, float* out_lp, float* out_phase, float* out_eloc,
+                    uint32_t* out_flags, const Shape& sh, hipStream_t s) {
+  const unsigned grid = (unsigned)((sh.B + 127) / 128);
+  if (sh.C > 1)
+    hipLaunchKernelGGL(k_combine<true>, dim3(grid), dim3(128), 0, s, ld, sd, g, lapd, dflags, r, R, Z, out_lp,
+                       out_phase, out_eloc, out_flags, sh);
+  else
+    hipLaunchKernelGGL(k_combine<false>, dim3(grid), dim3(128), 0, s, ld, sd, g, lapd, dflags, r, R, Z, out_lp,
+                       out_phase, out_eloc, out_flags, sh);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Metropolis-Hastings pieces (mcmc.py:175-180, 345-387).
+// ------------------------------------------------------------------------------------------------
+struct ChainScalars {
+  float stepsize;
+  float acc_rate;
+  int32_t step_nr;
+  int32_t pad;
+};
+
+__global__ void k_fill_i32(int32_t* p, int32_t v, int count) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) p[t] = v;
+}
+void launch_fill_i32(int32_t* p, int32_t v, int count, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_i32, dim3((count + 255) / 256), dim3(256), 0, s, p, v, count);
+}
+
+// r' = r + normal * stepsize
+__global__ void k_propose(const float* __restrict__ r, const float* __restrict__ normals,
+                          const ChainScalars* __restrict__ chain, float* __restrict__ r_prop, int count) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) r_prop[t] = r[t] + normals[t] * chain->stepsize;
+}
+void launch_propose(const float* r, const float* normals, const void* chain, float* r_prop, int count, hipStream_t s) {
+  hipLaunchKernelGGL(k_propose, dim3((count + 255) / 256), dim3(256), 0, s, r, normals, (const ChainScalars*)chain,
+                     r_prop, count);
+}
+
+// One 64-thread group per walker: lane 0 decides, all lanes copy the accepted coordinates.
+// accept = exp(lp' - lp) > u  or  old age >= max_age;  age <- 0 / age + 1;  lp and r selected.
+__global__ void k_accept(float* __restrict__ r, const float* __restrict__ r_prop, float* __restrict__ lp,
+                         const float* __restrict__ lp_prop, const float* __restrict__ uniforms,
+                         int32_t* __restrict__ age, int max_age, int32_t* __restrict__ acc_count,
+                         uint32_t* __restrict__ flags, int B, int n) {
+  const int b = blockIdx.x;
+  if (b >= B) return;
+  const int lane = threadIdx.x;
+  const float p_accept = expf(lp_prop[b] - lp[b]);
+  const bool by_ratio = p_accept > uniforms[b];
+  const int old_age = age[b];
+  const bool forced = old_age >= max_age;
+  const bool acc = by_ratio || forced;
+  for (int c = lane; c < n * 3; c += 64)
+    if (acc) r[(long)b * n * 3 + c] = r_prop[(long)b * n * 3 + c];
+  __syncthreads();
+  if (lane == 0) {
+    age[b] = acc ? 0 : old_age + 1;
+    if (acc) lp[b] = lp_prop[b];
+    if (acc) atomicAdd(acc_count, 1);
+    if (flags) flags[b] = (forced && !by_ratio) ? DWF_FORCED : 0u;
+  }
+}
+void launch_accept(float* r, const float* r_prop, float* lp, const float* lp_prop, const float* uniforms,
+                   int32_t* age, int max_age, int32_t* acc_count, uint32_t* flags, int B, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_accept, dim3(B), dim3(64), 0, s, r, r_prop, lp, lp_prop, uniforms, age, max_age, acc_count,
+                     flags, B, n);
+}
+
+// acc = mean(accept); step_nr += 1; acc_rate = 0.9 acc_rate + 0.1 acc; every `interval` steps the
+// stepsize is scaled by 1/1.05 or 1.05 from the OLD acc_rate and clipped (mcmc.py:367-387).
+__global__ void k_chain_update(ChainScalars* chain, const int32_t* acc_count, int B, float* out_acc, int interval,
+                               float target, float lo, float hi) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const float acc = (float)(*acc_count) / (float)B;
+  const float old_rate = chain->acc_rate;
+  const float old_step = chain->stepsize;
+  chain->step_nr += 1;
+  chain->acc_rate = 0.9f * old_rate + 0.1f * acc;
+  if (interval > 0 && (chain->step_nr % interval) == 0) {
+    float s = (old_rate < target) ? old_step / 1.05f : old_step * 1.05f;
+    s = fminf(fmaxf(s, lo), hi);
+    chain->stepsize = s;
+  }
+  if (out_acc) *out_acc = acc;
+}
+void launch_chain_update(void* chain, const int32_t* acc_count, int B, float* out_acc, int interval, float target,
+                         float lo, float hi, hipStream_t s) {
+  hipLaunchKernelGGL(k_chain_update, dim3(1), dim3(64), 0, s, (ChainScalars*)chain, acc_count, B, out_acc, interval,
+                     target, lo, hi);
+}
+
+}  // namespace dwmi

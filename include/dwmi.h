@@ -1,0 +1,121 @@
+/*
+ * dwmi.h — C ABI of the MI355X-native (gfx950) walker-parallel VMC inner loop.
+ *
+ * The library evaluates the FermiNet-style wavefunction of the reference (mdsunivie/deeperwin), its
+ * Metropolis-Hastings chain and its local energy on one GPU.  Every entry point below replaces one
+ * Python call of the reference; the reference interface is cited (file:line under src/deeperwin/).
+ *
+ * Conventions
+ *  - All array arguments are DEVICE pointers owned by the caller, row-major, float32 unless stated.
+ *    r [B, n_el, 3]; R [n_ion, 3]; Z [n_ion] (charges as float).
+ *  - Calls are ordered on the given HIP stream and never synchronise the host.
+ *  - A model handle may be used from one stream at a time; different handles are independent.
+ *  - Errors return a non-zero dw_status; dw_last_error() gives a thread-local message.  Non-finite
+ *    numbers are values, not errors: they are reported per walker through out_flags.
+ *  - The handle owns its parameter copy and a workspace that grows on demand (hipMalloc).
+ */
+#ifndef DWMI_H
+#define DWMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int dw_status;
+#define DW_OK 0
+#define DW_ERR_INVALID 1
+#define DW_ERR_HIP 2
+#define DW_ERR_STATE 3
+
+typedef struct dw_model dw_model;
+typedef void* dw_stream; /* a hipStream_t; NULL is the default stream */
+
+/* Precision forms.  DW_FP32: every GEMM in FP32 on MFMA.  DW_BF16X3: embedding and orbital GEMMs on
+ * BF16 MFMA with FP32 accumulation, operands split into hi+lo BF16 pairs (three products).
+ * DW_BF16: one BF16 product (fastest, least accurate). */
+#define DW_FP32 0
+#define DW_BF16X3 1
+#define DW_BF16 2
+
+/* Per-walker flag bits written to out_flags. */
+#define DW_FLAG_NONFINITE 1u     /* log|psi|^2 or E_loc is NaN/inf                          */
+#define DW_FLAG_FP64_DET 2u      /* at least one determinant was refactorised in FP64       */
+#define DW_FLAG_FORCED_ACCEPT 4u /* MCMC: accepted because walker age reached max_age       */
+
+typedef struct {
+  int32_t n_el;    /* electrons                                  */
+  int32_t n_up;    /* spin-up electrons (the first n_up rows)    */
+  int32_t n_ion;   /* nuclei                                     */
+  int32_t n_det;   /* full determinants                          */
+  int32_t width1;  /* 1-el stream width (256)                    */
+  int32_t width2;  /* 2-el stream width (32)                     */
+  int32_t n_iter;  /* FermiNet iterations (4)                    */
+  int32_t precision;       /* DW_FP32 / DW_BF16X3 / DW_BF16       */
+  float fp64_cond_threshold; /* condition estimate above which a det is redone in FP64 (<=0: never) */
+} dw_model_desc;
+
+/* Create a model for one (n_el, n_up, n_ion) shape.  Replaces build_log_psi_squared's model
+ * construction (model/wavefunction.py:262-299). */
+dw_status dw_model_create(const dw_model_desc* desc, dw_model** out);
+dw_status dw_model_destroy(dw_model* m);
+
+/* Upload one parameter leaf from HOST memory (float32, row-major).  `module_path` is the haiku module
+ * path and `leaf` the leaf name (w, b, alpha_up, alpha_dn, weights_up, weights_dn); paths are matched
+ * by suffix, so 'wf/ferminet_embedding/h_el_0/linear_0' and '.../~/h_el_0/linear_0' are both accepted.
+ * Replaces handing `params` (a haiku dict) to log_psi_sqr (model/wavefunction.py:293; leaves at
+ * mlp.py:57-59, envelope_orbitals.py:101-104). */
+dw_status dw_model_set_param(dw_model* m, const char* module_path, const char* leaf, const float* host_data,
+                             const int64_t* shape, int32_t ndim);
+/* Returns 1 when every leaf has been set. */
+int32_t dw_model_ready(const dw_model* m);
+
+/* log|psi|^2 and phase per walker: log_psi_sqr(params, n_up, n_dn, r, R, Z, fixed_params)
+ * -> (phase, log_psi_sqr)  (model/wavefunction.py:118-134, 293). */
+dw_status dw_logpsi(dw_model* m, const float* r, int32_t n_walkers, const float* R, const float* Z,
+                    float* out_log_psi_sqr, float* out_phase, dw_stream stream);
+
+/* Local energy per walker (kinetic via a forward-mode Laplacian over 3*n_el directions, plus Coulomb
+ * potential): get_local_energy(params, spin_state, r, R, Z, fixed_params) -> E_loc[B]
+ * (hamiltonian.py:272-291, 198-269, 17-39).  out_log_psi_sqr may be NULL; out_flags may be NULL. */
+dw_status dw_local_energy(dw_model* m, const float* r, int32_t n_walkers, const float* R, const float* Z,
+                          float* out_eloc, float* out_log_psi_sqr, uint32_t* out_flags, dw_stream stream);
+
+/* Chain scalars kept on the device so a run of steps needs no host synchronisation. */
+typedef struct {
+  float stepsize;
+  float acc_rate;
+  int32_t step_nr;
+  int32_t _pad;
+} dw_chain_scalars;
+
+/* n_steps Metropolis-Hastings steps with Gaussian proposals and SUPPLIED draws:
+ * normals [n_steps, B, n_el, 3], uniforms [n_steps, B].  r, log_psi_sqr and walker_age are updated in
+ * place; chain (device dw_chain_scalars) carries stepsize / acc_rate / step_nr; out_acc [n_steps]
+ * receives each step's acceptance fraction (may be NULL).  log_psi_sqr is re-evaluated on entry.
+ * Replaces MetropolisHastingsMonteCarlo.run_mcmc_steps (mcmc.py:325-327, 345-406). */
+dw_status dw_mcmc_steps(dw_model* m, float* r, int32_t n_walkers, const float* R, const float* Z,
+                        float* log_psi_sqr, int32_t* walker_age, const float* normals, const float* uniforms,
+                        int32_t n_steps, dw_chain_scalars* chain, int32_t max_age,
+                        int32_t stepsize_update_interval, float target_acceptance, float min_stepsize,
+                        float max_stepsize, float* out_acc, uint32_t* out_flags, dw_stream stream);
+
+/* Bytes of device workspace the model holds after sizing for n_walkers in the given mode
+ * (0 = log|psi| / MCMC, 1 = local energy). */
+size_t dw_workspace_size(const dw_model* m, int32_t n_walkers, int32_t mode);
+
+/* Cap on walkers processed per internal pass of dw_local_energy (0 = no cap).  Bounds the workspace. */
+dw_status dw_model_set_chunk(dw_model* m, int32_t max_walkers_per_pass);
+
+/* Thread-local message of the last failing call on this thread. */
+const char* dw_last_error(void);
+
+/* Library version string. */
+const char* dw_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DWMI_H */

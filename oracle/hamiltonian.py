@@ -1,0 +1,80 @@
+"""CPU restatement of the reference's local energy — TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
+
+* ``potential_energy``  <- get_potential_energy and helpers (src/deeperwin/hamiltonian.py:17-39)
+* ``kinetic_energy``    <- get_kinetic_energy, default forward_lap=False branch (hamiltonian.py:234-266):
+  g = grad(log psi^2); the Hessian diagonal comes from 3n Jacobian-vector products of g with unit vectors
+  (the reference uses jax.linearize + fori_loop; torch.func.jvp under vmap computes the same products);
+  E_kin = -1/2 (1/4 |g|^2 + 1/2 tr H).
+* ``local_energy``      <- build_local_energy (hamiltonian.py:272-291), walkers processed in chunks of
+  ``max_batch_size`` = 64 like folx.batched_vmap.
+"""
+from __future__ import annotations
+
+from typing import Dict
+
+import numpy as np
+import torch
+
+from . import wavefunction as wf
+
+
+def potential_energy(r: torch.Tensor, R: torch.Tensor, Z: torch.Tensor) -> torch.Tensor:
+    """E_pot = sum_{i<j} 1/|r_i - r_j| - sum_{i,I} Z_I/|r_i - R_I| + sum_{I<J} Z_I Z_J/|R_I - R_J|.
+    The el-el and ion-ion sums add the identity to the distance matrix and keep the strict upper
+    triangle (hamiltonian.py:17-31); the el-ion sum uses get_el_ion_distance_matrix (hamiltonian.py:34-36)."""
+    dt = r.dtype
+    R = R.to(dt)
+    Z = Z.to(dt)
+    n = r.shape[-2]
+    eye = torch.eye(n, dtype=dt)
+    d_ee = torch.linalg.norm(r[..., :, None, :] - r[..., None, :, :], dim=-1)
+    e_ee = torch.triu(1.0 / (d_ee + eye), diagonal=1).sum(dim=(-2, -1))
+    d_ei = torch.linalg.norm(r[..., :, None, :] - R[None, :, :], dim=-1)
+    e_ei = -(Z / d_ei).sum(dim=(-2, -1))
+    nI = R.shape[0]
+    eyeI = torch.eye(nI, dtype=dt)
+    d_II = torch.linalg.norm(R[:, None, :] - R[None, :, :], dim=-1)
+    e_II = torch.triu((Z[:, None] * Z[None, :]) / (d_II + eyeI), diagonal=1).sum()
+    return e_ee + e_ei + e_II
+
+
+def kinetic_energy(params: Dict, n_up: int, n_dn: int, r: torch.Tensor, R: torch.Tensor,
+                   dtype=torch.float64) -> torch.Tensor:
+    """Per-walker kinetic energy for r [B, n, 3] by forward-over-reverse Hessian-vector products."""
+    n = n_up + n_dn
+    R = R.to(dtype)
+
+    def f(x):  # x: [3n] -> log psi^2 of one walker
+        return wf.log_psi_sqr(params, n_up, n_dn, x.reshape(n, 3), R, dtype=dtype)[1]
+
+    grad_f = torch.func.grad(f)
+    eye = torch.eye(3 * n, dtype=dtype)
+
+    def one_walker(x):
+        g = grad_f(x)
+        hess_cols = torch.func.vmap(lambda v: torch.func.jvp(grad_f, (x,), (v,))[1])(eye)
+        lap = torch.diagonal(hess_cols).sum()
+        return -0.5 * (0.25 * (g ** 2).sum() + 0.5 * lap)
+
+    return torch.func.vmap(one_walker)(r.reshape(r.shape[0], 3 * n).to(dtype))
+
+
+def local_energy(params: Dict, n_up: int, n_dn: int, r, R, Z, max_batch_size: int = 64,
+                 dtype=torch.float64) -> torch.Tensor:
+    r = torch.as_tensor(np.asarray(r)) if not isinstance(r, torch.Tensor) else r
+    R = torch.as_tensor(np.asarray(R)) if not isinstance(R, torch.Tensor) else R
+    Z = torch.as_tensor(np.asarray(Z)) if not isinstance(Z, torch.Tensor) else Z
+    r = r.to(dtype)
+    out = []
+    for s in range(0, r.shape[0], max_batch_size):
+        rc = r[s:s + max_batch_size]
+        out.append(kinetic_energy(params, n_up, n_dn, rc, R, dtype=dtype) + potential_energy(rc, R.to(dtype), Z))
+    return torch.cat(out)
+
+
+def mean_and_var(e_loc) -> tuple:
+    """E_mean = nanmean(E_loc), E_var = nanmean(|E_loc - E_mean|^2) in float64
+    (optimization/evaluation.py:143-151, single device so pmean is the identity)."""
+    e = np.asarray(e_loc, dtype=np.float64)
+    m = np.nanmean(e)
+    return float(m), float(np.nanmean(np.abs(e - m) ** 2))

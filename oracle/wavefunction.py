@@ -1,0 +1,150 @@
+"""CPU restatement of the reference's FermiNet-style log|psi|^2 — TEST INFRASTRUCTURE ONLY (see
+``oracle/__init__.py``: tests, smoke() and bench's cpu_baseline leg are the only importers).
+
+Written in PyTorch CPU ops so that ``torch.func`` can differentiate it the way the reference's JAX code
+is differentiated.  Every function cites the reference lines (under ``src/deeperwin/``) it restates.
+Parameters use the haiku-style dict of ``deeperwin_amd.params`` (suffix-matched module paths).
+"""
+from __future__ import annotations
+
+import math
+from typing import Dict, Tuple
+
+import numpy as np
+import torch
+
+LOG_EPSILON = 1e-8  # model/wavefunction.py:64
+
+
+def _leaf(params: Dict, suffix: str, leaf: str, dtype, device="cpu") -> torch.Tensor:
+    hits = []
+    for path, leaves in params.items():
+        norm = "/".join(s for s in path.split("/") if s != "~")
+        if (norm == suffix or norm.endswith("/" + suffix)) and leaf in leaves:
+            hits.append(leaves[leaf])
+    if len(hits) != 1:
+        raise KeyError(f"{suffix}/{leaf}: {len(hits)} matches")
+    v = hits[0]
+    if isinstance(v, torch.Tensor):
+        return v.to(dtype=dtype, device=device)
+    return torch.as_tensor(np.asarray(v), dtype=dtype, device=device)
+
+
+def el_ion_features(r: torch.Tensor, R: torch.Tensor):
+    """diff[i, I] = r_i - R_I, dist = |diff|   (utils/utils.py:262-273).
+    Pair features [dist, dx, dy, dz]          (model/input_features.py:41-68)."""
+    diff = r[..., :, None, :] - R[..., None, :, :]
+    dist = torch.linalg.norm(diff, dim=-1)
+    return diff, dist, torch.cat([dist[..., None], diff], dim=-1)
+
+
+def el_el_features(r: torch.Tensor):
+    """diff[i, j] = r_j - r_i; the diagonal is padded by +1 before the norm and multiplied by 0 after,
+    so diagonal distances are exactly 0 with zero gradient (utils/utils.py:287-305, full=True).
+    Pair features [dist, dx, dy, dz] (model/input_features.py:41-68)."""
+    n = r.shape[-2]
+    eye = torch.eye(n, dtype=r.dtype)
+    diff = r[..., None, :, :] - r[..., :, None, :]
+    dist = torch.linalg.norm(diff + eye[..., None], dim=-1) * (1.0 - eye)
+    return diff, dist, torch.cat([dist[..., None], diff], dim=-1)
+
+
+def residual_update(update: torch.Tensor, old: torch.Tensor) -> torch.Tensor:
+    """(old + update)/sqrt(2) only when the shapes agree (model/mlp.py:13-16)."""
+    if old is None or old.shape != update.shape:
+        return update
+    return (old + update) / math.sqrt(2.0)
+
+
+def mlp_layer(x: torch.Tensor, w: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    """One-layer MLP with residual=True: y = tanh(x W + b), then residual_update(y, x)
+    (model/mlp.py:45-68)."""
+    y = torch.tanh(x @ w + b)
+    return residual_update(y, x)
+
+
+def symmetric_features(h: torch.Tensor, h_ee: torch.Tensor, n_up: int) -> torch.Tensor:
+    """concat[h, mean_up(h), mean_dn(h) tiled over electrons, mean_{j<n_up} h_ee[i, j],
+    mean_{j>=n_up} h_ee[i, j]]; the pair means include the self pair
+    (model/embeddings/ferminet_embedding.py:52-94, default flags)."""
+    n = h.shape[-2]
+    up = h[..., :n_up, :].mean(dim=-2, keepdim=True)
+    dn = h[..., n_up:, :].mean(dim=-2, keepdim=True)
+    up_t = up.expand(*h.shape[:-2], n, h.shape[-1])
+    dn_t = dn.expand(*h.shape[:-2], n, h.shape[-1])
+    pairs_up = h_ee[..., :n_up, :].mean(dim=-2)
+    pairs_dn = h_ee[..., n_up:, :].mean(dim=-2)
+    return torch.cat([h, up_t, dn_t, pairs_up, pairs_dn], dim=-1)
+
+
+def ferminet_embedding(params: Dict, h: torch.Tensor, h_ee: torch.Tensor, n_up: int, n_iter: int):
+    """Python-unrolled FermiNet iterations (model/embeddings/ferminet_embedding.py:207-267): symmetric
+    features -> 1-el MLP 'h_el_{i}'; then (except after the last iteration) 2-el MLP 'h_el_el_{i}'.
+    The 1-el MLP's residual never applies because its input is the wider symmetric feature vector."""
+    dt = h.dtype
+    for i in range(n_iter):
+        f = symmetric_features(h, h_ee, n_up)
+        h = mlp_layer(f, _leaf(params, f"h_el_{i}/linear_0", "w", dt), _leaf(params, f"h_el_{i}/linear_0", "b", dt))
+        if i == n_iter - 1:
+            break
+        h_ee = mlp_layer(h_ee, _leaf(params, f"h_el_el_{i}/linear_0", "w", dt),
+                         _leaf(params, f"h_el_el_{i}/linear_0", "b", dt))
+    return h
+
+
+def envelope_orbitals(params: Dict, h: torch.Tensor, dist_ei: torch.Tensor, n_up: int, n_dn: int, n_det: int):
+    """Full-determinant envelope orbitals (model/orbitals/envelope_orbitals.py:39-127):
+    bf_s = h_s W_s (no bias) reshaped to [D, n_s, n]; env_s[i, c] = sum_I w_s[I, c] exp(-softplus(alpha_s[I, c])
+    * |r_i - R_I|); mo_s = env_s * bf_s.  register_scale_and_shift is an identity in the forward pass."""
+    dt = h.dtype
+    n = n_up + n_dn
+    out = []
+    for spin, sl in (("up", slice(0, n_up)), ("dn", slice(n - n_dn, n))):
+        w = _leaf(params, f"bf_{spin}/linear_0", "w", dt)
+        alpha = _leaf(params, "envelope_orbitals", f"alpha_{spin}", dt)
+        weights = _leaf(params, "envelope_orbitals", f"weights_{spin}", dt)
+        bf = h[..., sl, :] @ w                                        # [..., n_s, D*n]
+        bf = bf.reshape(*bf.shape[:-1], n_det, n).transpose(-3, -2)   # [..., D, n_s, n]
+        d = dist_ei[..., sl, :, None]                                 # [..., n_s, I, 1]
+        env = (weights * torch.exp(-torch.nn.functional.softplus(alpha) * d)).sum(dim=-2)  # [..., n_s, D*n]
+        env = env.reshape(*env.shape[:-1], n_det, n).movedim(-2, -3)  # [..., D, n_s, n]
+        out.append(env * bf)
+    return out[0], out[1]
+
+
+def sum_of_determinants(mo_up: torch.Tensor, mo_dn: torch.Tensor):
+    """slogdet of the stacked [D, n, n] matrices, log-sum-exp over dets with the max shift and
+    LOG_EPSILON inside the log (model/wavefunction.py:63-83).  Returns (phase, log_psi_sqr)."""
+    mo = torch.cat([mo_up, mo_dn], dim=-2)
+    sign, logdet = torch.linalg.slogdet(mo)
+    shift = torch.amax(logdet, dim=-1, keepdim=True)   # amax spreads ties like jnp.max's JVP
+    psi = (torch.exp(logdet - shift) * sign).sum(dim=-1)
+    log_psi_sqr = 2.0 * (torch.log(torch.abs(psi) + LOG_EPSILON) + shift.squeeze(-1))
+    phase = torch.where(psi < 0, torch.full_like(psi, math.pi), torch.zeros_like(psi))
+    return phase, log_psi_sqr
+
+
+def log_psi_sqr(params: Dict, n_up: int, n_dn: int, r, R, Z=None, fixed_params=None, n_det: int = None,
+                dtype=torch.float64) -> Tuple[torch.Tensor, torch.Tensor]:
+    """Wavefunction.__call__ (model/wavefunction.py:118-134) for the FermiNet-style model:
+    InputPreprocessor (input_features.py:112-303: el-ion features concatenated ion-major into h_one,
+    el-el features, ion-ion features unused) -> FermiNetEmbedding -> EnvelopeOrbitals -> sum of dets.
+    Jastrow and el-el cusp are off for this model (configuration.py:913-923)."""
+    r = torch.as_tensor(r, dtype=dtype) if not isinstance(r, torch.Tensor) else r.to(dtype)
+    R = torch.as_tensor(np.asarray(R), dtype=dtype) if not isinstance(R, torch.Tensor) else R.to(dtype)
+    n = n_up + n_dn
+    n_iter = 0
+    while True:
+        try:
+            _leaf(params, f"h_el_{n_iter}/linear_0", "b", dtype)
+        except KeyError:
+            break
+        n_iter += 1
+    if n_det is None:
+        n_det = _leaf(params, "bf_up/linear_0", "w", dtype).shape[1] // n
+    _, dist_ei, f_ei = el_ion_features(r, R)
+    _, _, f_ee = el_el_features(r)
+    h = f_ei.reshape(*f_ei.shape[:-2], -1)           # [..., n, 4 * n_ion], ion-major
+    h = ferminet_embedding(params, h, f_ee, n_up, n_iter)
+    mo_up, mo_dn = envelope_orbitals(params, h, dist_ei, n_up, n_dn, n_det)
+    return sum_of_determinants(mo_up, mo_dn)

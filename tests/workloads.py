@@ -1,0 +1,34 @@
+"""Seeded test inputs shared by CPU and GPU tests (shapes follow the LiH / N2 / C4H2-like workloads)."""
+from __future__ import annotations
+
+import numpy as np
+
+from deeperwin_amd.params import ModelShape, init_params
+from deeperwin_amd.physics import generate_el_ion_mapping, geometry_c4h2, geometry_lih, geometry_n2, init_walkers_gaussian
+
+
+def system(name: str, n_det: int = None, seed: int = 0):
+    if name == "lih":
+        R, Z = geometry_lih()
+        D = n_det or 4
+    elif name == "n2":
+        R, Z = geometry_n2()
+        D = n_det or 16
+    elif name == "c4h2":
+        R, Z = geometry_c4h2(3)
+        D = n_det or 16
+    elif name == "he":
+        R, Z = np.zeros((1, 3)), np.array([2])
+        D = n_det or 2
+    else:
+        raise KeyError(name)
+    n = int(Z.sum())
+    u = (n + 1) // 2
+    shape = ModelShape(n, u, len(Z), D)
+    params = init_params(shape, seed)
+    mapping = generate_el_ion_mapping(R, Z, n, u)
+    return shape, params, R, Z, mapping
+
+
+def walkers(R, mapping, B: int, seed: int = 0):
+    return init_walkers_gaussian(np.random.default_rng(seed), B, R, mapping).astype(np.float32)
