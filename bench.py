@@ -1,0 +1,202 @@
+#!/usr/bin/env python
+"""Throughput of the walker-parallel VMC inner loop on one MI355X.
+
+One step = the reference's evaluation unit for the N2 workload: 20 Metropolis-Hastings steps (Gaussian
+proposals, stepsize 0.02 bohr, draws generated on the device from a seeded generator) followed by the
+local energy of every walker with the full Laplacian over 3 n_el = 42 directions, for 2048 walkers of
+N2 (Z = [7, 7], 2.07 bohr, 14 electrons, 7 up) with a FermiNet-style net of 4 iterations, 256 / 32 widths,
+16 full determinants, random init from seed 1.  Walkers start Gaussian (sigma 1 bohr) around the nuclei
+and the chain continues from step to step.
+
+value = walkers * steps / seconds  ("walker local-energies/s"; one walker local energy = the chain steps
+plus one Laplacian evaluation for that walker).
+
+Extra fields: `roofline` for the dominant kernel (the 1-el embedding GEMM in Laplacian mode, timed with
+HIP events on its launch stream inside the timed region), `cpu_baseline` (the CPU oracle in FP32, same
+per-walker work on a bounded walker sample) and `accuracy` (FP64 oracle on a walker sample of the final
+state).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "walker local-energies/s at |dE|<=0.1 mHa vs FP64; p50/p99 request latency"
+PEAK_TFLOPS = {"fp32": 157.3, "bf16x3": 2500.0, "bf16": 2500.0}
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3", "bf16"])
+    ap.add_argument("--walkers", type=int, default=2048)
+    ap.add_argument("--mcmc-steps", type=int, default=20)
+    ap.add_argument("--cpu-walkers", type=int, default=32)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    return ap.parse_args()
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc summary, if any."""
+    f = ROOT / "profiles" / "gemm_1el_lap_n2_traffic.txt"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (walkers shard across GPUs "
+              "without any per-step exchange beyond one scalar, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from deeperwin_amd._native import TIMER_FORWARD, TIMER_GEMM_1EL_LAP, TIMER_LOCAL_ENERGY, NativeModel
+    from deeperwin_amd.mcmc import make_chain
+    from deeperwin_amd.params import ModelShape, init_params
+    from deeperwin_amd.physics import generate_el_ion_mapping, geometry_n2, init_walkers_gaussian
+
+    if not torch.cuda.is_available():
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    R, Z = geometry_n2()
+    n, u, D, B, S = 14, 7, 16, args.walkers, args.mcmc_steps
+    shape = ModelShape(n_el=n, n_up=u, n_ion=2, n_det=D)
+    params = init_params(shape, seed=1)
+    mapping = generate_el_ion_mapping(R, Z, n, u)
+    r0 = init_walkers_gaussian(np.random.default_rng(1), B, R, mapping)
+
+    dev = "cuda"
+    model = NativeModel(shape, args.precision)
+    model.set_params(params)
+    r = torch.as_tensor(r0, dtype=torch.float32, device=dev).contiguous()
+    Rd = torch.as_tensor(R, dtype=torch.float32, device=dev)
+    Zd = torch.as_tensor(Z, dtype=torch.float32, device=dev)
+    lp = torch.full((B,), -1000.0, device=dev)
+    age = torch.zeros(B, dtype=torch.int32, device=dev)
+    chain = make_chain(0.02)
+    e = torch.empty(B, device=dev)
+    flags = torch.empty(B, dtype=torch.int32, device=dev)
+    acc = torch.empty(S, device=dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(1)
+
+    def step():
+        normals = torch.randn((S, B, n, 3), generator=gen, device=dev)
+        uniforms = torch.rand((S, B), generator=gen, device=dev)
+        model.mcmc_steps(r, Rd, Zd, lp, age, normals, uniforms, chain, 100, 100, 0.5, 1e-2, 1.0, out_acc=acc)
+        model.local_energy(r, Rd, Zd, e, None, flags)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    model.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    dt = (t1 - t0) / args.steps
+    value = B / dt
+
+    gemm_ms, gemm_n = model.timer(TIMER_GEMM_1EL_LAP)
+    fwd_ms, fwd_n = model.timer(TIMER_FORWARD)
+    lap_ms, lap_n = model.timer(TIMER_LOCAL_ENERGY)
+    model.set_timing(False)
+    C = 3 * n + 2
+    M = B * n * C
+    K = shape.width1 + 2 * shape.width2
+    flops = 2.0 * M * K * shape.width1
+    avg_s = (gemm_ms / max(gemm_n, 1)) / 1e3
+    achieved = flops / avg_s / 1e12
+    peak = PEAK_TFLOPS[args.precision]
+    roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(achieved / peak, 4), "traffic": traffic_from_profiles(),
+                "kernel": "1-el embedding GEMM, Laplacian mode (k_gemm_f32<2>)",
+                "flops_per_launch": flops, "avg_launch_ms": round(avg_s * 1e3, 4), "launches": gemm_n}
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "walker local-energies/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
+        "config": {"workload": "N2 (Z=[7,7], 2.07 bohr), 14 electrons (7 up), FermiNet 4 iterations 256/32 widths, "
+                               f"16 full dets, random init seed 1; {B} walkers; {S} MCMC steps (stepsize 0.02) + "
+                               f"E_loc with the full Laplacian over {3 * n} directions per step",
+                   "walkers": B, "mcmc_steps": S, "n_el": n, "n_det": D, "precision": args.precision,
+                   "parallelism": "single GPU (walker-parallel)"},
+        "roofline": roofline,
+        "breakdown": {"forward_pass_ms": round(fwd_ms / max(fwd_n, 1), 4),
+                      "local_energy_pass_ms": round(lap_ms / max(lap_n, 1), 4),
+                      "mcmc_walker_steps_per_s": round(B * S / ((S + 1) * fwd_ms / max(fwd_n, 1) / 1e3), 1),
+                      "eloc_only_walkers_per_s": round(B / (lap_ms / max(lap_n, 1) / 1e3), 1)},
+    }
+
+    e_dev = e.double().cpu().numpy()
+    r_fin = r.cpu().numpy().astype(np.float64)
+    out["energy_mean"] = float(np.nanmean(e_dev))
+
+    if not args.no_accuracy:
+        import torch as _t
+
+        from oracle import hamiltonian as oh
+
+        k = 16
+        idx = np.linspace(0, B - 1, k).astype(int)
+        e64 = oh.local_energy(params, u, n - u, r_fin[idx], R, Z).numpy()
+        de = e_dev[idx] - e64
+        out["accuracy"] = {"sample_walkers": k, "vs": "FP64 CPU oracle on the same walkers",
+                           "median_abs_dE": float(np.median(np.abs(de))), "max_abs_dE": float(np.max(np.abs(de))),
+                           "dE_mean_mHa": float(np.mean(de) * 1e3)}
+
+    if not args.no_cpu_baseline:
+        import torch as _t
+
+        from oracle import hamiltonian as oh
+        from oracle import mcmc as om
+        from oracle import wavefunction as owf
+
+        cores = len(os.sched_getaffinity(0))
+        _t.set_num_threads(cores)
+        k = args.cpu_walkers
+        rng = np.random.default_rng(7)
+        rc = r0[:k].astype(np.float32)
+        normals = rng.standard_normal((S, k, n, 3)).astype(np.float32)
+        uniforms = rng.uniform(size=(S, k)).astype(np.float32)
+
+        def f(x):
+            return owf.log_psi_sqr(params, u, n - u, _t.tensor(x), R, dtype=_t.float32)[1].numpy()
+
+        c0 = time.perf_counter()
+        st = om.ChainState(r=rc, log_psi_sqr=np.full(k, -1000.0, np.float32), walker_age=np.zeros(k, np.int64),
+                           stepsize=0.02)
+        st, _, _ = om.mcmc_steps(f, st, normals, uniforms, max_age=100, update_interval=100)
+        oh.local_energy(params, u, n - u, st.r, R, Z, max_batch_size=64, dtype=_t.float32)
+        c1 = time.perf_counter()
+        out["cpu_baseline"] = {"value": round(k / (c1 - c0), 3), "unit": "walker local-energies/s", "cores": cores,
+                               "kind": "port",
+                               "sample": f"{k} walkers of the same workload: {S} MCMC steps + E_loc (3n HVPs, "
+                                         f"chunks of 64) in FP32 PyTorch CPU; {c1 - c0:.1f} s",
+                               "baseline_only": True}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

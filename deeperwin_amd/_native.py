@@ -20,6 +20,7 @@ LIB_PATH = Path(os.environ.get("DWMI_LIB", Path(__file__).resolve().parent / "li
 DW_FP32, DW_BF16X3, DW_BF16 = 0, 1, 2
 PRECISIONS = {"fp32": DW_FP32, "bf16x3": DW_BF16X3, "bf16": DW_BF16}
 FLAG_NONFINITE, FLAG_FP64_DET, FLAG_FORCED_ACCEPT = 1, 2, 4
+TIMER_GEMM_1EL_LAP, TIMER_DET_LAP, TIMER_FORWARD, TIMER_LOCAL_ENERGY = 0, 1, 2, 3
 
 # Exported symbols and their ctypes signatures (argtypes, restype).
 _vp, _i32, _f32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_float
@@ -43,6 +44,9 @@ SIGNATURES = {
                        _vp, _vp], ctypes.c_int),
     "dw_workspace_size": ([_vp, _i32, _i32], ctypes.c_size_t),
     "dw_model_set_chunk": ([_vp, _i32], ctypes.c_int),
+    "dw_model_set_timing": ([_vp, _i32], ctypes.c_int),
+    "dw_model_timer_query": ([_vp, _i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)],
+                             ctypes.c_int),
     "dw_last_error": ([], ctypes.c_char_p),
     "dw_version": ([], ctypes.c_char_p),
 }
@@ -129,6 +133,16 @@ class NativeModel:
 
     def set_chunk(self, walkers: int) -> None:
         _check(self.lib.dw_model_set_chunk(self.handle, int(walkers)))
+
+    def set_timing(self, enable: bool) -> None:
+        _check(self.lib.dw_model_set_timing(self.handle, 1 if enable else 0))
+
+    def timer(self, kind: int):
+        """(total milliseconds, launches) of one timed region kind since timing was enabled."""
+        ms = ctypes.c_double()
+        cnt = ctypes.c_int32()
+        _check(self.lib.dw_model_timer_query(self.handle, int(kind), ctypes.byref(ms), ctypes.byref(cnt)))
+        return ms.value, cnt.value
 
     def workspace_bytes(self, n_walkers: int, mode: int) -> int:
         return int(self.lib.dw_workspace_size(self.handle, int(n_walkers), int(mode)))

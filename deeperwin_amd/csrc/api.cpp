@@ -114,8 +114,21 @@ struct dw_model {
   std::vector<DevBuf> X;
   DevBuf MU, V, Zb, H, O, A, ldd, sdd, gd, lapd, dfl, rprop, lpprop, cnt;
   std::vector<std::pair<std::string, std::vector<int64_t>>> expected;
+  // measurement hooks
+  bool timing = false;
+  struct TimerRec {
+    hipEvent_t a, b;
+    int kind;
+  };
+  std::vector<TimerRec> timers;
+  std::vector<hipEvent_t> event_pool;
 
   ~dw_model() {
+    for (auto& t : timers) {
+      (void)hipEventDestroy(t.a);
+      (void)hipEventDestroy(t.b);
+    }
+    for (auto e : event_pool) (void)hipEventDestroy(e);
     for (auto* v : {&Wmain, &Wmu, &bias, &Wmain_hi, &Wmain_lo, &Wmu_hi, &Wmu_lo, &X})
       for (auto& b : *v) b.release();
     for (DevBuf* b : {&W2e, &b2e, &Wo[0], &Wo[1], &Wo_hi[0], &Wo_hi[1], &Wo_lo[0], &Wo_lo[1], &kappa, &wenv, &MU, &V,
@@ -283,6 +296,39 @@ size_t ws_bytes(const dw_model* m, long B, int C) {
   return tot;
 }
 
+hipEvent_t get_event(dw_model* m) {
+  if (!m->event_pool.empty()) {
+    hipEvent_t e = m->event_pool.back();
+    m->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+// Bracket a region with events on stream s when timing is on.
+struct Timed {
+  dw_model* m;
+  int kind;
+  hipStream_t s;
+  hipEvent_t a = nullptr;
+  // kind < 0: not timed
+  Timed(dw_model* m_, int kind_, hipStream_t s_) : m(m_), kind(kind_), s(s_) {
+    if (m->timing && kind >= 0) {
+      a = get_event(m);
+      (void)hipEventRecord(a, s);
+    }
+  }
+  ~Timed() {
+    if (a) {
+      hipEvent_t b = get_event(m);
+      (void)hipEventRecord(b, s);
+      m->timers.push_back({a, b, kind});
+    }
+  }
+};
+
 // One pass of the wavefunction over B walkers with C channels (C == 1: value; C == 3n+2: Laplacian).
 dw_status run_pass(dw_model* m, const float* r, long B, const float* R, const float* Z, int C, hipStream_t s,
                    float* out_lp, float* out_phase, float* out_eloc, uint32_t* out_flags) {
@@ -290,6 +336,7 @@ dw_status run_pass(dw_model* m, const float* r, long B, const float* R, const fl
   if (st) return st;
   Shape sh{(int)B, m->n, m->u, m->I, m->D, C, m->W1, m->W2, m->L};
   const long rows = B * m->n * C;
+  Timed pass_timer(m, C > 1 ? DW_TIMER_LOCAL_ENERGY : DW_TIMER_FORWARD, s);
   const GemmPrec pr = m->prec;
   std::vector<float*> X(m->L);
   for (int l = 0; l < m->L; ++l) X[l] = m->X[l].f();
@@ -320,6 +367,7 @@ dw_status run_pass(dw_model* m, const float* r, long B, const float* R, const fl
       gemm(pr, X[l], m->Kx[l], m->Wmain[l].f(), m->Wmain_hi[l].h(), m->Wmain_lo[l].h(), m->Kx[l], outp, ldo,
            (int)rows, m->W1, m->Kx[l], em, s);
     } else {
+      Timed t(m, l >= 1 ? DW_TIMER_GEMM_1EL_LAP : -1, s);
       gemm(pr, X[l], m->Kx[l], m->Wmain[l].f(), m->Wmain_hi[l].h(), m->Wmain_lo[l].h(), m->Kx[l], m->Zb.f(), m->W1,
            (int)rows, m->W1, m->Kx[l], em, s);
       launch_tanh_rule(m->Zb.f(), outp, ldo, last ? 1 : 0, sh, s);
@@ -333,7 +381,10 @@ dw_status run_pass(dw_model* m, const float* r, long B, const float* R, const fl
   gemm(pr, m->H.f() + up_rows * m->W1, m->W1, m->Wo[1].f(), m->Wo_hi[1].h(), m->Wo_lo[1].h(), m->W1,
        m->O.f() + up_rows * m->NO, m->NO, (int)dn_rows, m->NO, m->W1, e0, s);
   launch_envelope(m->O.f(), m->NO, r, R, m->kappa.f(), m->wenv.f(), m->A.f(), sh, s);
-  launch_det(m->A.f(), m->ldd.f(), m->sdd.f(), m->gd.f(), m->lapd.f(), m->dfl.u(), m->d.fp64_cond_threshold, sh, s);
+  {
+    Timed t(m, C > 1 ? DW_TIMER_DET_LAP : -1, s);
+    launch_det(m->A.f(), m->ldd.f(), m->sdd.f(), m->gd.f(), m->lapd.f(), m->dfl.u(), m->d.fp64_cond_threshold, sh, s);
+  }
   launch_combine(m->ldd.f(), m->sdd.f(), m->gd.f(), m->lapd.f(), m->dfl.u(), r, R, Z, out_lp, out_phase, out_eloc,
                  out_flags, sh, s);
   HIP_TRY(hipGetLastError());
@@ -346,6 +397,34 @@ extern "C" {
 
 const char* dw_last_error(void) { return g_err.c_str(); }
 const char* dw_version(void) { return "dwmi 0.1 (gfx950)"; }
+
+dw_status dw_model_set_timing(dw_model* m, int32_t enable) {
+  if (!m) return fail(DW_ERR_INVALID, "null model");
+  for (auto& t : m->timers) {
+    m->event_pool.push_back(t.a);
+    m->event_pool.push_back(t.b);
+  }
+  m->timers.clear();
+  m->timing = enable != 0;
+  return DW_OK;
+}
+
+dw_status dw_model_timer_query(dw_model* m, int32_t kind, double* total_ms, int32_t* launches) {
+  if (!m || !total_ms || !launches) return fail(DW_ERR_INVALID, "null argument");
+  double tot = 0.0;
+  int cnt = 0;
+  for (auto& t : m->timers) {
+    if (t.kind != kind) continue;
+    HIP_TRY(hipEventSynchronize(t.b));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, t.a, t.b));
+    tot += ms;
+    ++cnt;
+  }
+  *total_ms = tot;
+  *launches = cnt;
+  return DW_OK;
+}
 
 dw_status dw_model_create(const dw_model_desc* desc, dw_model** out) {
   if (!desc || !out) return fail(DW_ERR_INVALID, "null argument");

@@ -56,16 +56,16 @@ void launch_features0(const float* r, const float* R, float* X0, int Kx0, const 
 }
 
 // ------------------------------------------------------------------------------------------------
-// Fused two-electron stream.  One workgroup per (walker b, electron i); 256 threads = 8 partner slots
-// x 32 features.  For each partner j the pair features [|r_j - r_i|, r_j - r_i] (utils.py:287-305) go
-// through the 2-el MLPs 'h_el_el_l' (tanh, residual (x+y)/sqrt(2) when widths agree; mlp.py:13-16,
-// 45-68) carrying the value, the 3 tangents along d = r_j - r_i and the Laplacian.  Because a pair
-// feature depends on r_j - r_i only, d/dr_j = +T and d/dr_i = -T, and the Laplacian over all 3 n
-// directions sees each |T_c|^2 twice.  The spin means over partners (ferminet_embedding.py:79-81;
-// the self pair is included) are written into the pair columns of every layer input X_l:
+// Fused two-electron stream.  One workgroup per (walker b, group of 8 electrons i); 256 threads =
+// 8 electrons x 32 features.  Every thread walks the partners j in order: the pair features
+// [|r_j - r_i|, r_j - r_i] (utils.py:287-305) go through the 2-el MLPs 'h_el_el_l' (tanh, residual
+// (x+y)/sqrt(2) when widths agree; mlp.py:13-16, 45-68) carrying the value, the 3 tangents along
+// d = r_j - r_i and the Laplacian.  A pair feature depends on r_j - r_i only, so d/dr_j = +T and
+// d/dr_i = -T and the Laplacian over all 3 n directions sees each |T_c|^2 twice.  The spin means over
+// partners (ferminet_embedding.py:79-81; the self pair is included) are accumulated in registers in a
+// fixed order and written into the pair columns of every layer input X_l:
 //   value / Laplacian rows : mean over j in the spin block
 //   tangent row (e, c)     : T_c[i, e] / n_s(e) in block s(e)  (e != i);  -sum_j T_c[i, j] / n_s (e == i)
-// Partial sums are kept per slot and reduced in a fixed order, so results are deterministic.
 // ------------------------------------------------------------------------------------------------
 struct TwoElOut {
   float* X[MAXL];
@@ -73,156 +73,173 @@ struct TwoElOut {
   int Kh[MAXL];
 };
 
-template <bool LAP>
+template <bool LAP, int L>
 __global__ __launch_bounds__(256) void k_twoel(const float* __restrict__ r, TwoElOut out,
                                                const float* __restrict__ W2e, const float* __restrict__ b2e,
                                                Shape sh) {
-  const int L = sh.L, n = sh.n, u = sh.u, C = sh.C;
-  const long bi = blockIdx.x;
-  const int i = (int)(bi % n);
-  const long b = bi / n;
-  const int tid = threadIdx.x, slot = tid >> 5, f = tid & 31;
   constexpr int NCH = LAP ? 5 : 1;
-  __shared__ float Ws[MAXL - 1][32][32];
-  __shared__ float bs[MAXL - 1][32];
-  __shared__ float xin[8][5][32];
-  __shared__ float part[8][MAXL][2][5][32];
+  constexpr int NL2 = (L > 1) ? L - 1 : 1;
+  const int n = sh.n, u = sh.u, C = sh.C;
+  const int groups = (n + 7) >> 3;
+  const long b = blockIdx.x / groups;
+  const int il = threadIdx.x >> 5, f = threadIdx.x & 31;
+  const int i = (int)(blockIdx.x % groups) * 8 + il;
+  const bool active = i < n;
+  const long bi = b * n + (active ? i : 0);
+  __shared__ float xs[8][NCH][32];
 
-  for (int idx = tid; idx < (L - 1) * 32 * 32; idx += 256) (&Ws[0][0][0])[idx] = W2e[idx];
-  for (int idx = tid; idx < (L - 1) * 32; idx += 256) (&bs[0][0])[idx] = b2e[idx];
-  for (int idx = tid; idx < L * 2 * 5 * 32; idx += 256) {
-    const int ff = idx & 31, rest = idx >> 5;  // rest = (l * 2 + s) * 5 + ch
-    const int ch = rest % 5, ls = rest / 5;
+  float w[NL2][32], bb[NL2];
 #pragma unroll
-    for (int sl = 0; sl < 8; ++sl) part[sl][ls >> 1][ls & 1][ch][ff] = 0.f;
+  for (int l2 = 0; l2 < L - 1; ++l2) {
+    bb[l2] = b2e[l2 * 32 + f];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) w[l2][k] = W2e[(l2 * 32 + k) * 32 + f];
   }
+  float acc0[L][NCH], acc1[L][NCH];
+#pragma unroll
+  for (int l = 0; l < L; ++l)
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) acc0[l][c] = acc1[l][c] = 0.f;
+
   const float* rb = r + b * n * 3;
-  const float rix = rb[i * 3 + 0], riy = rb[i * 3 + 1], riz = rb[i * 3 + 2];
+  const float rix = active ? rb[i * 3 + 0] : 0.f, riy = active ? rb[i * 3 + 1] : 0.f,
+              riz = active ? rb[i * 3 + 2] : 0.f;
   const float inv_up = 1.f / (float)u, inv_dn = 1.f / (float)(n - u);
   const float rs2 = 0.70710678118654752440f;
-  __syncthreads();
 
-  for (int j0 = 0; j0 < n; j0 += 8) {
-    const int j = j0 + slot;
-    const bool valid = j < n;
+  for (int j = 0; j < n; ++j) {
     const int sj = (j < u) ? 0 : 1;
     const float inv_s = sj == 0 ? inv_up : inv_dn;
-    // ---- raw pair features (layer-0 2-el input) ----
-    float xv = 0.f, xt0 = 0.f, xt1 = 0.f, xt2 = 0.f, xl = 0.f;
-    if (valid && j != i && f < 4) {
+    const bool offdiag = active && j != i;
+    float x[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) x[c] = 0.f;
+    if (offdiag && f < 4) {
       const float dx = rb[j * 3 + 0] - rix, dy = rb[j * 3 + 1] - riy, dz = rb[j * 3 + 2] - riz;
       const float rho = sqrtf(dx * dx + dy * dy + dz * dz);
       if (f == 0) {
-        xv = rho;
-        xt0 = dx / rho;
-        xt1 = dy / rho;
-        xt2 = dz / rho;
-        xl = 4.f / rho;
+        x[0] = rho;
+        if constexpr (LAP) {
+          x[1] = dx / rho;
+          x[2] = dy / rho;
+          x[3] = dz / rho;
+          x[4] = 4.f / rho;
+        }
       } else {
-        xv = (f == 1) ? dx : (f == 2 ? dy : dz);
-        xt0 = (f == 1) ? 1.f : 0.f;
-        xt1 = (f == 2) ? 1.f : 0.f;
-        xt2 = (f == 3) ? 1.f : 0.f;
+        x[0] = (f == 1) ? dx : (f == 2 ? dy : dz);
+        if constexpr (LAP) {
+          x[1] = (f == 1) ? 1.f : 0.f;
+          x[2] = (f == 2) ? 1.f : 0.f;
+          x[3] = (f == 3) ? 1.f : 0.f;
+        }
       }
     }
-    if (valid && f < 4) {
-      part[slot][0][sj][0][f] += xv;
-      if (LAP) {
-        part[slot][0][sj][1][f] += xt0;
-        part[slot][0][sj][2][f] += xt1;
-        part[slot][0][sj][3][f] += xt2;
-        part[slot][0][sj][4][f] += xl;
-        if (j != i) {
-          const float tv[3] = {xt0, xt1, xt2};
+    if (sj == 0) {
 #pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            float* row = out.X[0] + ((bi * C) + 1 + 3 * j + c) * (long)out.Kx[0] + out.Kh[0];
-            row[sj * 4 + f] = tv[c] * inv_s;
-            row[(1 - sj) * 4 + f] = 0.f;
+      for (int c = 0; c < NCH; ++c) acc0[0][c] += x[c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) acc1[0][c] += x[c];
+    }
+    if constexpr (LAP) {
+      if (offdiag && f < 4) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          float* row = out.X[0] + (bi * C + 1 + 3 * j + c) * (long)out.Kx[0] + out.Kh[0];
+          row[sj * 4 + f] = x[1 + c] * inv_s;
+          row[(1 - sj) * 4 + f] = 0.f;
+        }
+      }
+    }
+#pragma unroll
+    for (int l2 = 0; l2 < L - 1; ++l2) {
+      const int kin = (l2 == 0) ? 4 : 32;
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) xs[il][c][f] = x[c];
+      __syncthreads();
+      float z[NCH];
+      z[0] = bb[l2];
+#pragma unroll
+      for (int c = 1; c < NCH; ++c) z[c] = 0.f;
+#pragma unroll
+      for (int k4 = 0; k4 < 32; k4 += 4) {
+        if (k4 < kin) {
+#pragma unroll
+          for (int c = 0; c < NCH; ++c) {
+            const float4 xv = *(const float4*)&xs[il][c][k4];
+            z[c] += xv.x * w[l2][k4] + xv.y * w[l2][k4 + 1] + xv.z * w[l2][k4 + 2] + xv.w * w[l2][k4 + 3];
           }
         }
       }
-    }
-    xin[slot][0][f] = xv;
-    if (LAP) {
-      xin[slot][1][f] = xt0;
-      xin[slot][2][f] = xt1;
-      xin[slot][3][f] = xt2;
-      xin[slot][4][f] = xl;
-    }
-    __syncthreads();
-    // ---- 2-el MLP layers ----
-    for (int l2 = 0; l2 < L - 1; ++l2) {
-      const int kin = (l2 == 0) ? 4 : 32;
-      float zv = bs[l2][f], z0 = 0.f, z1 = 0.f, z2 = 0.f, zl = 0.f;
-      for (int k = 0; k < kin; ++k) {
-        const float w = Ws[l2][k][f];
-        zv += xin[slot][0][k] * w;
-        if (LAP) {
-          z0 += xin[slot][1][k] * w;
-          z1 += xin[slot][2][k] * w;
-          z2 += xin[slot][3][k] * w;
-          zl += xin[slot][4][k] * w;
-        }
-      }
-      const float th = tanhf(zv);
-      const float d1 = 1.f - th * th, d2 = -2.f * th * d1;
-      float y[5];
+      const float th = tanhf(z[0]);
+      float y[NCH];
       y[0] = th;
-      if (LAP) {
-        y[1] = d1 * z0;
-        y[2] = d1 * z1;
-        y[3] = d1 * z2;
-        y[4] = d1 * zl + d2 * 2.f * (z0 * z0 + z1 * z1 + z2 * z2);
+      if constexpr (LAP) {
+        const float d1 = 1.f - th * th, d2 = -2.f * th * d1;
+        y[1] = d1 * z[1];
+        y[2] = d1 * z[2];
+        y[3] = d1 * z[3];
+        y[4] = d1 * z[4] + d2 * 2.f * (z[1] * z[1] + z[2] * z[2] + z[3] * z[3]);
       }
-      if (kin == 32) {  // residual: input and output widths agree
+      if (kin == 32) {
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) y[c] = (xin[slot][c][f] + y[c]) * rs2;
+        for (int c = 0; c < NCH; ++c) y[c] = (x[c] + y[c]) * rs2;
       }
-      __syncthreads();
 #pragma unroll
-      for (int c = 0; c < NCH; ++c) xin[slot][c][f] = y[c];
-      if (valid) {
-        const int l = l2 + 1;
+      for (int c = 0; c < NCH; ++c) x[c] = y[c];
+      if (sj == 0) {
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) part[slot][l][sj][c][f] += y[c];
-        if (LAP && j != i) {
+        for (int c = 0; c < NCH; ++c) acc0[l2 + 1][c] += y[c];
+      } else {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) acc1[l2 + 1][c] += y[c];
+      }
+      if constexpr (LAP) {
+        if (offdiag) {
+          const int l = l2 + 1;
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            float* row = out.X[l] + ((bi * C) + 1 + 3 * j + c) * (long)out.Kx[l] + out.Kh[l];
+            float* row = out.X[l] + (bi * C + 1 + 3 * j + c) * (long)out.Kx[l] + out.Kh[l];
             row[sj * 32 + f] = y[1 + c] * inv_s;
             row[(1 - sj) * 32 + f] = 0.f;
           }
         }
       }
-      __syncthreads();
     }
   }
-  __syncthreads();
-  // ---- reduce the 8 slots in a fixed order and write value / Laplacian / self-tangent rows ----
+  if (!active) return;
+#pragma unroll
   for (int l = 0; l < L; ++l) {
     const int Ke = (l == 0) ? 4 : 32;
-    for (int idx = tid; idx < 2 * NCH * Ke; idx += 256) {
-      const int ff = idx % Ke, rest = idx / Ke;
-      const int c = rest % NCH, s = rest / NCH;
-      float acc = 0.f;
+    if (f >= Ke) continue;
+    float* X = out.X[l];
+    const long Kx = out.Kx[l];
+    const int Kh = out.Kh[l];
 #pragma unroll
-      for (int sl = 0; sl < 8; ++sl) acc += part[sl][l][s][c][ff];
+    for (int s = 0; s < 2; ++s) {
       const float inv = (s == 0) ? inv_up : inv_dn;
-      long chrow;
-      float v;
-      if (c == 0) {
-        chrow = 0;
-        v = acc * inv;
-      } else if (c == 4) {
-        chrow = C - 1;
-        v = acc * inv;
-      } else {
-        chrow = 1 + 3 * i + (c - 1);
-        v = -acc * inv;
+      float a[NCH];
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) a[c] = (s == 0) ? acc0[l][c] : acc1[l][c];
+      X[(bi * C) * Kx + Kh + s * Ke + f] = a[0] * inv;
+      if constexpr (LAP) {
+        X[(bi * C + C - 1) * Kx + Kh + s * Ke + f] = a[4] * inv;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) X[(bi * C + 1 + 3 * i + c) * Kx + Kh + s * Ke + f] = -a[1 + c] * inv;
       }
-      out.X[l][((bi * C) + chrow) * (long)out.Kx[l] + out.Kh[l] + s * Ke + ff] = v;
     }
+  }
+}
+
+template <bool LAP>
+static void twoel_dispatch(const float* r, const TwoElOut& o, const float* W2e, const float* b2e, const Shape& sh,
+                           hipStream_t s) {
+  const unsigned grid = (unsigned)(sh.B * ((sh.n + 7) / 8));
+  switch (sh.L) {
+    case 2: hipLaunchKernelGGL((k_twoel<LAP, 2>), dim3(grid), dim3(256), 0, s, r, o, W2e, b2e, sh); break;
+    case 3: hipLaunchKernelGGL((k_twoel<LAP, 3>), dim3(grid), dim3(256), 0, s, r, o, W2e, b2e, sh); break;
+    default: hipLaunchKernelGGL((k_twoel<LAP, 4>), dim3(grid), dim3(256), 0, s, r, o, W2e, b2e, sh); break;
   }
 }
 
@@ -234,11 +251,10 @@ void launch_twoel(const float* r, float* const* X, const int* Kx, const int* Kh,
     o.Kx[l] = l < sh.L ? Kx[l] : 0;
     o.Kh[l] = l < sh.L ? Kh[l] : 0;
   }
-  const unsigned grid = (unsigned)(sh.B * sh.n);
   if (sh.C > 1)
-    hipLaunchKernelGGL(k_twoel<true>, dim3(grid), dim3(256), 0, s, r, o, W2e, b2e, sh);
+    twoel_dispatch<true>(r, o, W2e, b2e, sh, s);
   else
-    hipLaunchKernelGGL(k_twoel<false>, dim3(grid), dim3(256), 0, s, r, o, W2e, b2e, sh);
+    twoel_dispatch<false>(r, o, W2e, b2e, sh, s);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -373,173 +389,300 @@ void launch_envelope(const float* O, int ldo, const float* r, const float* R, co
 }
 
 // ------------------------------------------------------------------------------------------------
-// Determinants: one workgroup per (walker, det).  Partial-pivot LU in LDS (first maximal |pivot| wins,
-// as LAPACK getrf), sign and log|det| (jnp.linalg.slogdet, wavefunction.py:69).  In Laplacian mode the
-// inverse is formed and for every tangent direction k: g_k = tr(M^-1 dM_k), q_k = tr((M^-1 dM_k)^2);
-// the Laplacian of log|det| is tr(M^-1 Lap M) - sum_k q_k.  A condition proxy max|U_pp| / min|U_pp| is
-// reported through dflags so that flagged determinants can be refactored in FP64.
+// Determinants, in FP64.  One wavefront per (walker, det) matrix; lane j holds column j in registers.
+// Partial-pivot LU (first maximal |pivot| wins, as LAPACK getrf): the pivot search runs in the pivot
+// column's lane, the multipliers are broadcast with readlane, every lane updates its own column.
+// sign and log|det| as jnp.linalg.slogdet (wavefunction.py:69).  In Laplacian mode the inverse is
+// formed column per lane (forward / back substitution against the broadcast factors) and stored in
+// LDS; the tangent channels are then processed several per pass (lane = (channel, column)):
+//   Y_k = M^-1 dM_k,  g_k = tr Y_k,  q_k = tr(Y_k^2),   Lap log|det| = tr(M^-1 Lap M) - sum_k q_k.
+// The 1-norm condition number |M|_1 |M^-1|_1 (Laplacian mode) or max|U_pp| / min|U_pp| (value mode)
+// is compared with `cond_thr` and reported through dflags.
 // ------------------------------------------------------------------------------------------------
-template <bool LAP>
-__global__ __launch_bounds__(256) void k_det(const float* __restrict__ A, float* __restrict__ ld,
-                                             float* __restrict__ sd, float* __restrict__ g, float* __restrict__ lapd,
-                                             uint32_t* __restrict__ dflags, float cond_thr, Shape sh) {
-  extern __shared__ float smem[];
-  const int n = sh.n, P = n + 1, C = sh.C;
-  float* M = smem;
-  float* Mi = M + n * P;
-  float* T = Mi + n * P;
-  float* lcol = T + n * P;
-  float* red = lcol + 64;
-  int* perm = (int*)(red + 32);
-  __shared__ int piv_sh;
-  __shared__ int sing_sh;
-  const long bd = blockIdx.x;
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const long nn = (long)n * n;
-  const float* Ab = A + bd * C * nn;
-  for (int e = tid; e < n * n; e += 256) M[(e / n) * P + (e % n)] = Ab[e];
-  if (tid < n) perm[tid] = tid;
-  if (tid == 0) sing_sh = 0;
-  __syncthreads();
-  int swaps = 0;
-  for (int p = 0; p < n; ++p) {
-    if (wid == 0) {
-      float best = -1.f;
-      int bidx = p;
-      const int rr = p + lane;
-      if (rr < n) {
-        best = fabsf(M[rr * P + p]);
-        bidx = rr;
-      }
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double wave_max_d(double v) {
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        const float ob = __shfl_xor(best, o, 64);
-        const int oi = __shfl_xor(bidx, o, 64);
-        if (ob > best || (ob == best && oi < bidx)) {
-          best = ob;
-          bidx = oi;
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_min_d(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// In-register LU of the column-per-lane matrix.  Returns the number of row swaps; perm[] (uniform)
+// records the row order; mydiag receives U_jj in lane j.
+template <int NMAX>
+__device__ __forceinline__ int lu_columns(double (&col)[NMAX], int (&perm)[NMAX], double& mydiag, bool& sing, int n,
+                                          int lane) {
+  int swaps = 0;
+#pragma unroll
+  for (int i = 0; i < NMAX; ++i) perm[i] = i;
+#pragma unroll
+  for (int p = 0; p < NMAX; ++p) {
+    if (p < n) {
+    double best = -1.0;
+    int bidx = p;
+#pragma unroll
+    for (int i = p; i < NMAX; ++i) {
+      if (i < n) {
+        const double a = fabs(col[i]);
+        if (a > best) {
+          best = a;
+          bidx = i;
         }
       }
-      if (lane == 0) {
-        piv_sh = bidx;
-        if (!(best > 0.f)) sing_sh = 1;
-      }
     }
-    __syncthreads();
-    const int piv = piv_sh;
+    const int piv = __builtin_amdgcn_readlane(bidx, p);
+    const double bval = readlane_d(best, p);
+    if (!(bval > 0.0)) sing = true;
     if (piv != p) {
-      for (int j = tid; j < n; j += 256) {
-        const float tmp = M[p * P + j];
-        M[p * P + j] = M[piv * P + j];
-        M[piv * P + j] = tmp;
-      }
-      if (tid == 0) {
-        const int tp = perm[p];
-        perm[p] = perm[piv];
-        perm[piv] = tp;
-      }
       ++swaps;
+#pragma unroll
+      for (int i = p + 1; i < NMAX; ++i) {
+        if (i == piv) {
+          const double t = col[p];
+          col[p] = col[i];
+          col[i] = t;
+          const int tp = perm[p];
+          perm[p] = perm[i];
+          perm[i] = tp;
+        }
+      }
     }
-    __syncthreads();
-    const float pivot = M[p * P + p];
-    for (int rr = p + 1 + tid; rr < n; rr += 256) {
-      const float l = M[rr * P + p] / pivot;
-      lcol[rr] = l;
-      M[rr * P + p] = l;
+    const double pivot = readlane_d(col[p], p);
+    if (lane == p) {
+      mydiag = col[p];
+#pragma unroll
+      for (int i = p + 1; i < NMAX; ++i)
+        if (i < n) col[i] = col[i] / pivot;
     }
-    __syncthreads();
-    const int m = n - p - 1;
-    for (int e = tid; e < m * m; e += 256) {
-      const int rr = p + 1 + e / m, cc = p + 1 + e % m;
-      M[rr * P + cc] -= lcol[rr] * M[p * P + cc];
+#pragma unroll
+    for (int i = p + 1; i < NMAX; ++i) {
+      if (i < n) {
+        const double li = readlane_d(col[i], p);
+        if (lane > p) col[i] -= li * col[p];
+      }
     }
-    __syncthreads();
+    }
   }
-  // sign / log|det| / condition proxy
-  if (tid == 0) {
-    float sgn = (swaps & 1) ? -1.f : 1.f;
-    float lsum = 0.f, umax = 0.f, umin = 3.4e38f;
-    for (int p = 0; p < n; ++p) {
-      const float up = M[p * P + p];
-      if (up < 0.f) sgn = -sgn;
-      const float a = fabsf(up);
-      lsum += logf(a);
-      umax = fmaxf(umax, a);
-      umin = fminf(umin, a);
-    }
-    if (sing_sh) {
-      sgn = 0.f;
-      lsum = -INFINITY;
-    }
-    ld[bd] = lsum;
-    sd[bd] = sgn;
-    uint32_t fl = 0;
-    if (cond_thr > 0.f && !(umax <= cond_thr * umin)) fl |= 1u;
-    dflags[bd] = fl;
+  return swaps;
+}
+
+template <int NMAX>
+__device__ __forceinline__ void finish_slogdet(double mydiag, int swaps, bool sing, int n, int lane, double& logabs,
+                                               double& sgn) {
+  const bool act = lane < n;
+  double lg = act ? log(fabs(mydiag)) : 0.0;
+  lg = wave_sum_d(lg);
+  const unsigned long long neg = __ballot(act && mydiag < 0.0);
+  const int nneg = __popcll(neg) + swaps;
+  sgn = (nneg & 1) ? -1.0 : 1.0;
+  logabs = lg;
+  if (sing) {
+    sgn = 0.0;
+    logabs = -INFINITY;
   }
-  if (!LAP) return;
-  // inverse: column j solves L U x = P e_j
-  for (int j = tid; j < n; j += 256) {
-    for (int p = 0; p < n; ++p) {
-      float y = (perm[p] == j) ? 1.f : 0.f;
-      for (int q = 0; q < p; ++q) y -= M[p * P + q] * Mi[q * P + j];
-      Mi[p * P + j] = y;
+}
+
+// Value mode: 4 matrices per 256-thread workgroup, no LDS.
+template <int NMAX>
+__global__ __launch_bounds__(256) void k_det_value(const float* __restrict__ A, float* __restrict__ ld,
+                                                   float* __restrict__ sd, uint32_t* __restrict__ dflags,
+                                                   float cond_thr, long nmat, int n, long stride) {
+  const int lane = threadIdx.x & 63;
+  const long bd = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (bd >= nmat) return;
+  const float* Ab = A + bd * stride;
+  const bool act = lane < n;
+  double col[NMAX];
+  int perm[NMAX];
+#pragma unroll
+  for (int i = 0; i < NMAX; ++i) col[i] = (act && i < n) ? (double)Ab[i * n + lane] : 0.0;
+  double mydiag = 1.0;
+  bool sing = false;
+  const int swaps = lu_columns<NMAX>(col, perm, mydiag, sing, n, lane);
+  double logabs, sgn;
+  finish_slogdet<NMAX>(mydiag, swaps, sing, n, lane, logabs, sgn);
+  const double umax = wave_max_d(act ? fabs(mydiag) : 0.0);
+  const double umin = wave_min_d(act ? fabs(mydiag) : 1e300);
+  if (lane == 0) {
+    ld[bd] = (float)logabs;
+    sd[bd] = (float)sgn;
+    dflags[bd] = (cond_thr > 0.f && !(umax <= (double)cond_thr * umin)) ? 1u : 0u;
+  }
+}
+
+// Laplacian mode: WPB matrices per workgroup, LDS holds M^-1 and the per-pass Y_k of every wave.
+template <int NMAX, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_det_lap(const float* __restrict__ A, float* __restrict__ ld,
+                                                      float* __restrict__ sd, float* __restrict__ g,
+                                                      float* __restrict__ lapd, uint32_t* __restrict__ dflags,
+                                                      float cond_thr, long nmat, int n, int C) {
+  constexpr int CPP = 64 / NMAX;  // channels per pass
+  extern __shared__ double dsm[];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  long bd = (long)blockIdx.x * WPB + wv;
+  const bool valid_mat = bd < nmat;
+  if (!valid_mat) bd = nmat - 1;  // keep the wave in the barriers; results are not stored
+  double* sMi = dsm + (size_t)wv * (NMAX * NMAX + 64 * NMAX);
+  double* sY = sMi + NMAX * NMAX;
+  const long nn = (long)n * n;
+  const float* Ab = A + bd * C * nn;
+  const bool act = lane < n;
+  const int nd = 3 * n;
+
+  double col[NMAX];
+  int perm[NMAX];
+  double csum = 0.0;
+#pragma unroll
+  for (int i = 0; i < NMAX; ++i) {
+    col[i] = (act && i < n) ? (double)Ab[i * n + lane] : 0.0;
+    csum += fabs(col[i]);
+  }
+  const double normA = wave_max_d(csum);
+  double mydiag = 1.0;
+  bool sing = false;
+  const int swaps = lu_columns<NMAX>(col, perm, mydiag, sing, n, lane);
+  double logabs, sgn;
+  finish_slogdet<NMAX>(mydiag, swaps, sing, n, lane, logabs, sgn);
+
+  // inverse, column `lane`: solve L y = P e_lane, then U x = y
+  double x[NMAX];
+#pragma unroll
+  for (int p = 0; p < NMAX; ++p) {
+    if (p < n) {
+      double y = (perm[p] == lane) ? 1.0 : 0.0;
+#pragma unroll
+      for (int q = 0; q < p; ++q) y -= readlane_d(col[p], q) * x[q];
+      x[p] = y;
+    } else {
+      x[p] = 0.0;
     }
-    for (int p = n - 1; p >= 0; --p) {
-      float x = Mi[p * P + j];
-      for (int q = p + 1; q < n; ++q) x -= M[p * P + q] * Mi[q * P + j];
-      Mi[p * P + j] = x / M[p * P + p];
+  }
+#pragma unroll
+  for (int p = NMAX - 1; p >= 0; --p) {
+    if (p < n) {
+      double y = x[p];
+#pragma unroll
+      for (int q = p + 1; q < NMAX; ++q)
+        if (q < n) y -= readlane_d(col[p], q) * x[q];
+      x[p] = y / readlane_d(col[p], p);
     }
+  }
+  double isum = 0.0;
+#pragma unroll
+  for (int a = 0; a < NMAX; ++a) isum += act ? fabs(x[a]) : 0.0;
+  const double normMi = wave_max_d(isum);
+  // M^-1 to LDS, row-major [a][e]
+  if (lane < NMAX) {
+#pragma unroll
+    for (int a = 0; a < NMAX; ++a) sMi[a * NMAX + lane] = act ? x[a] : 0.0;
   }
   __syncthreads();
-  const int nd = 3 * n;
-  float qacc = 0.f;
-  for (int ch = 1; ch <= nd; ++ch) {
-    const float* Tg = Ab + (long)ch * nn;
-    for (int e = tid; e < n * n; e += 256) T[(e / n) * P + (e % n)] = Tg[e];
-    __syncthreads();
-    // X = Mi * T  (reuse M)
-    for (int e = tid; e < n * n; e += 256) {
-      const int a = e / n, c = e % n;
-      float acc = 0.f;
-      for (int q = 0; q < n; ++q) acc += Mi[a * P + q] * T[q * P + c];
-      M[a * P + c] = acc;
+
+  const int cg = lane / NMAX, c = lane % NMAX;
+  const bool cact = c < n;
+  double qacc = 0.0;
+  for (int k0 = 0; k0 < nd; k0 += CPP) {
+    const int k = k0 + cg;
+    const bool kact = k < nd;
+    const float* Tg = Ab + (long)(1 + (kact ? k : 0)) * nn;
+    double jc[NMAX];
+#pragma unroll
+    for (int e = 0; e < NMAX; ++e) jc[e] = (cact && kact && e < n) ? (double)Tg[e * n + c] : 0.0;
+    double yc[NMAX];
+#pragma unroll
+    for (int a = 0; a < NMAX; ++a) {
+      double acc = 0.0;
+#pragma unroll
+      for (int e = 0; e < NMAX; e += 2) {
+        const double2 m2 = *(const double2*)&sMi[a * NMAX + e];
+        acc += m2.x * jc[e] + m2.y * jc[e + 1];
+      }
+      yc[a] = acc;
     }
+    double diag = 0.0;
+#pragma unroll
+    for (int a = 0; a < NMAX; ++a) diag += (a == c) ? yc[a] : 0.0;
+#pragma unroll
+    for (int a = 0; a < NMAX; ++a) sY[(cg * NMAX + a) * NMAX + c] = yc[a];
     __syncthreads();
-    float pd = 0.f, pq = 0.f;
-    for (int e = tid; e < n * n; e += 256) {
-      const int a = e / n, c = e % n;
-      const float x = M[a * P + c];
-      pq += x * M[c * P + a];
-      if (a == c) pd += x;
+    double qp = 0.0;
+#pragma unroll
+    for (int a = 0; a < NMAX; ++a) qp += sY[(cg * NMAX + c) * NMAX + a] * yc[a];
+    if (!cact) {
+      qp = 0.0;
+      diag = 0.0;
     }
-    const float gs = block_sum(pd, red);
-    const float qs = block_sum(pq, red + 16);
-    if (tid == 0) g[bd * nd + (ch - 1)] = gs;
-    qacc += qs;
+#pragma unroll
+    for (int o = NMAX / 2; o > 0; o >>= 1) {
+      qp += __shfl_xor(qp, o, NMAX);
+      diag += __shfl_xor(diag, o, NMAX);
+    }
+    if (c == 0 && kact) {
+      if (valid_mat) g[bd * nd + k] = (float)diag;
+      qacc += qp;
+    }
     __syncthreads();
   }
-  {
-    const float* Tl = Ab + (long)(C - 1) * nn;
-    float pt = 0.f;
-    for (int e = tid; e < n * n; e += 256) {
-      const int a = e / n, q = e % n;
-      pt += Mi[a * P + q] * Tl[q * n + a];
-    }
-    const float tr = block_sum(pt, red);
-    if (tid == 0) lapd[bd] = tr - qacc;
+  // sum q over the channel groups (group leaders hold partial sums)
+#pragma unroll
+  for (int o = NMAX; o < 64; o <<= 1) qacc += __shfl_xor(qacc, o, 64);
+  // tr(M^-1 Lap M): lane c takes column c of Lap M and row c of M^-1
+  const float* Tl = Ab + (long)(C - 1) * nn;
+  double tp = 0.0;
+  if (act) {
+#pragma unroll
+    for (int e = 0; e < NMAX; ++e)
+      if (e < n) tp += sMi[lane * NMAX + e] * (double)Tl[e * n + lane];
+  }
+  tp = wave_sum_d(tp);
+  if (lane == 0 && valid_mat) {
+    ld[bd] = (float)logabs;
+    sd[bd] = (float)sgn;
+    lapd[bd] = (float)(tp - qacc);
+    dflags[bd] = (cond_thr > 0.f && !(normA * normMi <= (double)cond_thr)) ? 1u : 0u;
   }
 }
 
 void launch_det(const float* A, float* ld, float* sd, float* g, float* lapd, uint32_t* dflags, float cond_thr,
                 const Shape& sh, hipStream_t s) {
   const int n = sh.n;
-  const size_t shm = (size_t)(3 * n * (n + 1) + 64 + 32 + 64) * sizeof(float);
-  const unsigned grid = (unsigned)(sh.B * sh.D);
-  if (sh.C > 1)
-    hipLaunchKernelGGL(k_det<true>, dim3(grid), dim3(256), shm, s, A, ld, sd, g, lapd, dflags, cond_thr, sh);
-  else
-    hipLaunchKernelGGL(k_det<false>, dim3(grid), dim3(256), shm, s, A, ld, sd, g, lapd, dflags, cond_thr, sh);
+  const long nmat = (long)sh.B * sh.D;
+  const long stride = (long)sh.C * n * n;
+  if (sh.C == 1) {
+    const unsigned grid = (unsigned)((nmat + 3) / 4);
+    if (n <= 16)
+      hipLaunchKernelGGL(k_det_value<16>, dim3(grid), dim3(256), 0, s, A, ld, sd, dflags, cond_thr, nmat, n, stride);
+    else if (n <= 32)
+      hipLaunchKernelGGL(k_det_value<32>, dim3(grid), dim3(256), 0, s, A, ld, sd, dflags, cond_thr, nmat, n, stride);
+    else
+      hipLaunchKernelGGL(k_det_value<64>, dim3(grid), dim3(256), 0, s, A, ld, sd, dflags, cond_thr, nmat, n, stride);
+    return;
+  }
+  if (n <= 16) {
+    constexpr int NM = 16, W = 4;
+    const size_t shm = (size_t)W * (NM * NM + 64 * NM) * sizeof(double);
+    hipLaunchKernelGGL((k_det_lap<NM, W>), dim3((unsigned)((nmat + W - 1) / W)), dim3(64 * W), shm, s, A, ld, sd, g,
+                       lapd, dflags, cond_thr, nmat, n, sh.C);
+  } else if (n <= 32) {
+    constexpr int NM = 32, W = 2;
+    const size_t shm = (size_t)W * (NM * NM + 64 * NM) * sizeof(double);
+    hipLaunchKernelGGL((k_det_lap<NM, W>), dim3((unsigned)((nmat + W - 1) / W)), dim3(64 * W), shm, s, A, ld, sd, g,
+                       lapd, dflags, cond_thr, nmat, n, sh.C);
+  } else {
+    constexpr int NM = 64, W = 1;
+    const size_t shm = (size_t)W * (NM * NM + 64 * NM) * sizeof(double);
+    hipLaunchKernelGGL((k_det_lap<NM, W>), dim3((unsigned)nmat), dim3(64 * W), shm, s, A, ld, sd, g, lapd, dflags,
+                       cond_thr, nmat, n, sh.C);
+  }
 }
 
 // ------------------------------------------------------------------------------------------------

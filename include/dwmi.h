@@ -109,6 +109,18 @@ size_t dw_workspace_size(const dw_model* m, int32_t n_walkers, int32_t mode);
 /* Cap on walkers processed per internal pass of dw_local_energy (0 = no cap).  Bounds the workspace. */
 dw_status dw_model_set_chunk(dw_model* m, int32_t max_walkers_per_pass);
 
+/* ---- Measurement hooks (no counterpart in the reference; used by the benchmark harness) ----
+ * When timing is enabled the library brackets selected launches with HIP events recorded on the
+ * launch stream; dw_model_timer_query synchronises those events and returns the summed duration and the
+ * number of bracketed launches of one region kind since timing was (re-)enabled. */
+#define DW_TIMER_GEMM_1EL_LAP 0 /* 1-el embedding GEMM, Laplacian mode, iterations >= 1 (one launch each) */
+#define DW_TIMER_DET_LAP 1      /* determinant kernel, Laplacian mode                                   */
+#define DW_TIMER_FORWARD 2      /* one value-only wavefunction pass (all its kernels)                   */
+#define DW_TIMER_LOCAL_ENERGY 3 /* one Laplacian pass (all its kernels)                                 */
+#define DW_TIMER_KINDS 4
+dw_status dw_model_set_timing(dw_model* m, int32_t enable);
+dw_status dw_model_timer_query(dw_model* m, int32_t kind, double* total_ms, int32_t* launches);
+
 /* Thread-local message of the last failing call on this thread. */
 const char* dw_last_error(void);
 

@@ -148,3 +148,38 @@ def log_psi_sqr(params: Dict, n_up: int, n_dn: int, r, R, Z=None, fixed_params=N
     h = ferminet_embedding(params, h, f_ee, n_up, n_iter)
     mo_up, mo_dn = envelope_orbitals(params, h, dist_ei, n_up, n_dn, n_det)
     return sum_of_determinants(mo_up, mo_dn)
+
+
+def slater_matrices(params: Dict, n_up: int, n_dn: int, r, R, n_det: int = None, dtype=torch.float64):
+    """Stacked [..., D, n, n] orbital matrices (Wavefunction.get_slater_matrices,
+    model/wavefunction.py:136-143, with the up and down rows concatenated as in wavefunction.py:66-68)."""
+    r = torch.as_tensor(np.asarray(r), dtype=dtype) if not isinstance(r, torch.Tensor) else r.to(dtype)
+    R = torch.as_tensor(np.asarray(R), dtype=dtype) if not isinstance(R, torch.Tensor) else R.to(dtype)
+    n = n_up + n_dn
+    n_iter = 0
+    while True:
+        try:
+            _leaf(params, f"h_el_{n_iter}/linear_0", "b", dtype)
+        except KeyError:
+            break
+        n_iter += 1
+    if n_det is None:
+        n_det = _leaf(params, "bf_up/linear_0", "w", dtype).shape[1] // n
+    _, dist_ei, f_ei = el_ion_features(r, R)
+    _, _, f_ee = el_el_features(r)
+    h = ferminet_embedding(params, f_ei.reshape(*f_ei.shape[:-2], -1), f_ee, n_up, n_iter)
+    mo_up, mo_dn = envelope_orbitals(params, h, dist_ei, n_up, n_dn, n_det)
+    return torch.cat([mo_up, mo_dn], dim=-2)
+
+
+def conditioning(params: Dict, n_up: int, n_dn: int, r, R) -> np.ndarray:
+    """Per-walker sensitivity of log|psi| to relative perturbations of the orbital matrices:
+    kappa = max_d cond_2(A_d) * sum_d |s_d e^{l_d - m}| / |sum_d s_d e^{l_d - m}| (FP64).
+    Rounding at unit roundoff u in the network perturbs log|psi|^2 by O(u kappa)."""
+    mo = slater_matrices(params, n_up, n_dn, r, R)
+    sign, logdet = torch.linalg.slogdet(mo)
+    m = torch.amax(logdet, dim=-1, keepdim=True)
+    e = sign * torch.exp(logdet - m)
+    cancel = e.abs().sum(-1) / e.sum(-1).abs()
+    cond = torch.linalg.cond(mo).amax(-1)
+    return (cond * cancel).numpy()

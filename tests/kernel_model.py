@@ -17,7 +17,7 @@ def _pair_means(val, T, Lp, u, lap):
     """val [B,n,n,F], T [B,n,n,3,F], Lp [B,n,n,F] -> rows [B, n, C, 2F]."""
     B, n, _, F = val.shape
     C = 3 * n + 2 if lap else 1
-    out = np.zeros((B, n, C, 2 * F))
+    out = np.zeros((B, n, C, 2 * F), val.dtype)
     blocks = [(0, u), (u, n)]
     for s, (j0, j1) in enumerate(blocks):
         ns = j1 - j0
@@ -47,16 +47,17 @@ def _tanh_rule(Z, lap):
     return out
 
 
-def run(params, n_up, r, R, Z, n_det, lap=True):
-    r = np.asarray(r, np.float64)
-    R = np.asarray(R, np.float64)
+def run(params, n_up, r, R, Z, n_det, lap=True, dtype=np.float64, det_dtype=np.float64):
+    """dtype: arithmetic of the network stages; det_dtype: arithmetic of the determinant stage."""
+    r = np.asarray(r, dtype)
+    R = np.asarray(R, dtype)
     B, n, _ = r.shape
     u, I, nd = n_up, R.shape[0], 3 * n
     C = nd + 2 if lap else 1
     # ---- layer-0 one-electron rows ----
     d = r[:, :, None, :] - R[None, None]
     rho = np.linalg.norm(d, axis=-1)
-    h = np.zeros((B, n, C, I, 4))
+    h = np.zeros((B, n, C, I, 4), dtype)
     h[:, :, 0, :, 0] = rho
     h[:, :, 0, :, 1:] = d
     if lap:
@@ -72,11 +73,11 @@ def run(params, n_up, r, R, Z, n_det, lap=True):
     rho_ee = np.linalg.norm(dd, axis=-1)
     rho_safe = np.where(eye, 1.0, rho_ee)
     val = np.concatenate([np.where(eye, 0.0, rho_ee)[..., None], dd], -1)
-    T = np.zeros((B, n, n, 3, 4))
+    T = np.zeros((B, n, n, 3, 4), dtype)
     for c in range(3):
         T[:, :, :, c, 0] = np.where(eye, 0.0, dd[..., c] / rho_safe)
         T[:, :, :, c, 1 + c] = np.where(eye, 0.0, 1.0)
-    Lp = np.zeros((B, n, n, 4))
+    Lp = np.zeros((B, n, n, 4), dtype)
     Lp[..., 0] = np.where(eye, 0.0, 4.0 / rho_safe)
     L = 0
     while True:
@@ -87,8 +88,8 @@ def run(params, n_up, r, R, Z, n_det, lap=True):
         L += 1
     pms = [_pair_means(val, T, Lp, u, lap)]
     for l2 in range(L - 1):
-        W = np.asarray(find_leaf(params, f"h_el_el_{l2}/linear_0", "w"), np.float64)
-        b = np.asarray(find_leaf(params, f"h_el_el_{l2}/linear_0", "b"), np.float64)
+        W = np.asarray(find_leaf(params, f"h_el_el_{l2}/linear_0", "w"), dtype)
+        b = np.asarray(find_leaf(params, f"h_el_el_{l2}/linear_0", "b"), dtype)
         z = val @ W + b
         zT = T @ W
         zL = Lp @ W
@@ -104,8 +105,8 @@ def run(params, n_up, r, R, Z, n_det, lap=True):
         pms.append(_pair_means(val, T, Lp, u, lap))
     # ---- one-electron layers with the spin-mean split ----
     for l in range(L):
-        W = np.asarray(find_leaf(params, f"h_el_{l}/linear_0", "w"), np.float64)
-        b = np.asarray(find_leaf(params, f"h_el_{l}/linear_0", "b"), np.float64)
+        W = np.asarray(find_leaf(params, f"h_el_{l}/linear_0", "w"), dtype)
+        b = np.asarray(find_leaf(params, f"h_el_{l}/linear_0", "b"), dtype)
         Kh = h.shape[-1]
         mu_up = h[:, :u].mean(1)   # [B, C, Kh]
         mu_dn = h[:, u:].mean(1)
@@ -115,16 +116,16 @@ def run(params, n_up, r, R, Z, n_det, lap=True):
         h = _tanh_rule(Zp, lap)
     # ---- orbitals ----
     NO = n_det * n
-    bf = np.zeros((B, n, C, NO))
-    Wu = np.asarray(find_leaf(params, "bf_up/linear_0", "w"), np.float64)
-    Wd = np.asarray(find_leaf(params, "bf_dn/linear_0", "w"), np.float64)
+    bf = np.zeros((B, n, C, NO), dtype)
+    Wu = np.asarray(find_leaf(params, "bf_up/linear_0", "w"), dtype)
+    Wd = np.asarray(find_leaf(params, "bf_dn/linear_0", "w"), dtype)
     bf[:, :u] = h[:, :u] @ Wu
     bf[:, u:] = h[:, u:] @ Wd
-    A = np.zeros((B, n_det, C, n, n))
+    A = np.zeros((B, n_det, C, n, n), dtype)
     for s, (i0, i1) in enumerate([(0, u), (u, n)]):
         sfx = "up" if s == 0 else "dn"
-        kap = np.log1p(np.exp(np.asarray(find_leaf(params, "envelope_orbitals", f"alpha_{sfx}"), np.float64)))
-        we = np.asarray(find_leaf(params, "envelope_orbitals", f"weights_{sfx}"), np.float64)
+        kap = np.log1p(np.exp(np.asarray(find_leaf(params, "envelope_orbitals", f"alpha_{sfx}"), dtype)))
+        we = np.asarray(find_leaf(params, "envelope_orbitals", f"weights_{sfx}"), dtype)
         for i in range(i0, i1):
             e = we[None] * np.exp(-kap[None] * rho[:, i, :, None])        # [B, I, NO]
             env = e.sum(1)
@@ -138,6 +139,7 @@ def run(params, n_up, r, R, Z, n_det, lap=True):
                 a[:, -1] = lenv * bf[:, i, 0] + env * bf[:, i, -1] + 2 * (genv * bf[:, i, 1 + 3 * i:4 + 3 * i]).sum(1)
             A[:, :, :, i, :] = a.reshape(B, C, n_det, n).transpose(0, 2, 1, 3)
     # ---- determinants ----
+    A = A.astype(det_dtype)
     sgn, ld = np.linalg.slogdet(A[:, :, 0])
     out = dict(ld=ld, sd=sgn)
     m = ld.max(1, keepdims=True)

@@ -1,11 +1,16 @@
 """GPU parity: the native path (through the C ABI) against the float64 CPU oracle on the same parameters,
 walker positions and supplied draws.
 
-Tolerances (FP32 device arithmetic vs the FP64 oracle):
-  log|psi|^2 : |d| <= 1e-3 + 1e-5 |ref|                       per walker
-  E_loc      : |d| <= 2e-3 + 2e-5 |ref|                       per walker
-  mean E_loc : |d| <= 1e-4 Ha (0.1 mHa)
-  MCMC       : accept decisions identical except where |exp(dlp) - u| < 1e-6 relative to the threshold
+Tolerances of the FP32 form (FP32 network, FP64 determinant stage) against the FP64 oracle.  Rounding
+at the FP32 unit roundoff u = 2^-24 in the network moves log|psi|^2 by O(u kappa), where kappa is the
+walker's conditioning (max_d cond_2(A_d) times the cancellation factor of the determinant sum, computed in
+FP64 by ``oracle.wavefunction.conditioning``).  Per walker:
+  log|psi|^2 : |d| <= 5e-5 max(1, |ref| / 10) + 4 u kappa
+  E_loc      : |d| <= (1 + |ref|) (1e-4 + 4 u kappa)
+and over the batch (catches errors that every walker shares):
+  median |d log|psi|^2| <= 5e-4,   median |d E_loc| <= 3e-3
+MCMC replay: with the same supplied draws >= 98% of walkers follow the oracle's chain exactly (a walker
+can only leave it at a decision within the FP32 noise of the threshold).
 """
 import numpy as np
 import pytest
@@ -35,6 +40,17 @@ def _oracle_lp(params, shape, r, R):
     return owf.log_psi_sqr(params, shape.n_up, shape.n_dn, torch.tensor(r, dtype=torch.float64), R)
 
 
+U32 = 2.0 ** -24
+
+
+def _tol_lp(ref, kappa):
+    return 5e-5 * np.maximum(1.0, np.abs(ref) / 10) + 4 * U32 * kappa
+
+
+def _tol_e(ref, kappa):
+    return (1.0 + np.abs(ref)) * (1e-4 + 4 * U32 * kappa)
+
+
 @pytest.mark.parametrize("name,B", [("lih", 64), ("n2", 32), ("c4h2", 8)])
 def test_logpsi_matches_oracle(cuda, name, B):
     shape, params, R, Z, mapping = system(name)
@@ -47,8 +63,11 @@ def test_logpsi_matches_oracle(cuda, name, B):
     ph_ref, lp_ref = _oracle_lp(params, shape, r, R)
     lp_ref = lp_ref.numpy()
     err = np.abs(lp.cpu().numpy() - lp_ref)
-    print(f"{name}: max |d log_psi_sqr| = {err.max():.3e}")
-    assert np.all(err <= 1e-3 + 1e-5 * np.abs(lp_ref))
+    kappa = owf.conditioning(params, shape.n_up, shape.n_dn, r, R)
+    print(f"{name}: max |d log_psi_sqr| = {err.max():.3e}, median {np.median(err):.2e}, max err/tol "
+          f"{np.max(err / _tol_lp(lp_ref, kappa)):.3f}")
+    assert np.all(err <= _tol_lp(lp_ref, kappa))
+    assert np.median(err) <= 5e-4
     np.testing.assert_array_equal(ph.cpu().numpy(), ph_ref.numpy().astype(np.float32))
 
 
@@ -64,11 +83,13 @@ def test_local_energy_matches_oracle(cuda, name, B):
     torch.cuda.synchronize()
     e_ref = oh.local_energy(params, shape.n_up, shape.n_dn, r.astype(np.float64), R, Z).numpy()
     err = np.abs(e.cpu().numpy() - e_ref)
-    print(f"{name}: E_loc ref {e_ref[:4]} max |dE| = {err.max():.3e}, mean dE = {abs(e.cpu().numpy().mean() - e_ref.mean()):.3e}")
-    assert np.all(err <= 2e-3 + 2e-5 * np.abs(e_ref))
-    assert abs(float(e.cpu().numpy().astype(np.float64).mean()) - float(e_ref.mean())) <= 1e-4 * max(1.0, B / 8)
+    kappa = owf.conditioning(params, shape.n_up, shape.n_dn, r, R)
+    print(f"{name}: E_loc ref {e_ref[:4]} max |dE| = {err.max():.3e}, median {np.median(err):.2e}, "
+          f"max err/tol {np.max(err / _tol_e(e_ref, kappa)):.3f}")
+    assert np.all(err <= _tol_e(e_ref, kappa))
+    assert np.median(err) <= 3e-3
     lp_ref = _oracle_lp(params, shape, r, R)[1].numpy()
-    assert np.all(np.abs(lp.cpu().numpy() - lp_ref) <= 1e-3 + 1e-5 * np.abs(lp_ref))
+    assert np.all(np.abs(lp.cpu().numpy() - lp_ref) <= _tol_lp(lp_ref, kappa))
     assert int(fl.cpu().numpy().max()) == 0
 
 
