@@ -112,7 +112,7 @@ struct dw_model {
   DevBuf W2e, b2e, Wo[2], Wo_hi[2], Wo_lo[2], kappa, wenv;
   // workspace
   std::vector<DevBuf> X;
-  DevBuf MU, V, Zb, H, O, A, ldd, sdd, gd, lapd, dfl, rprop, lpprop, cnt;
+  DevBuf MU, V, H, O, A, ldd, sdd, gd, lapd, dfl, rprop, lpprop, cnt;
   std::vector<std::pair<std::string, std::vector<int64_t>>> expected;
   // measurement hooks
   bool timing = false;
@@ -132,7 +132,7 @@ struct dw_model {
     for (auto* v : {&Wmain, &Wmu, &bias, &Wmain_hi, &Wmain_lo, &Wmu_hi, &Wmu_lo, &X})
       for (auto& b : *v) b.release();
     for (DevBuf* b : {&W2e, &b2e, &Wo[0], &Wo[1], &Wo_hi[0], &Wo_hi[1], &Wo_lo[0], &Wo_lo[1], &kappa, &wenv, &MU, &V,
-                      &Zb, &H, &O, &A, &ldd, &sdd, &gd, &lapd, &dfl, &rprop, &lpprop, &cnt})
+                      &H, &O, &A, &ldd, &sdd, &gd, &lapd, &dfl, &rprop, &lpprop, &cnt})
       b->release();
   }
 };
@@ -254,15 +254,17 @@ dw_status prepare(dw_model* m) {
   return DW_OK;
 }
 
+int rows_per_electron(int C) { return C > 1 ? pad_to(C, 16) : 1; }
+
 // Size the workspace for B walkers with C channels.
 dw_status ensure_ws(dw_model* m, long B, int C) {
-  const long rows = B * m->n * C;
+  const int Cp = rows_per_electron(C);
+  const long rows = B * m->n * Cp;
   for (int l = 0; l < m->L; ++l) HIP_TRY(m->X[l].ensure((size_t)rows * m->Kx[l] * 4));
   int kmmax = 0;
   for (int l = 0; l < m->L; ++l) kmmax = std::max(kmmax, m->Km[l]);
-  HIP_TRY(m->MU.ensure((size_t)B * C * kmmax * 4));
-  HIP_TRY(m->V.ensure((size_t)B * C * m->W1 * 4));
-  if (C > 1) HIP_TRY(m->Zb.ensure((size_t)rows * m->W1 * 4));
+  HIP_TRY(m->MU.ensure((size_t)B * Cp * kmmax * 4));
+  HIP_TRY(m->V.ensure((size_t)B * Cp * m->W1 * 4));
   HIP_TRY(m->H.ensure((size_t)rows * m->W1 * 4));
   HIP_TRY(m->O.ensure((size_t)rows * m->NO * 4));
   HIP_TRY(m->A.ensure((size_t)B * m->D * C * m->n * m->n * 4));
@@ -280,15 +282,15 @@ dw_status ensure_ws(dw_model* m, long B, int C) {
 }
 
 size_t ws_bytes(const dw_model* m, long B, int C) {
-  const long rows = B * m->n * C;
+  const int Cp = rows_per_electron(C);
+  const long rows = B * m->n * Cp;
   size_t tot = 0;
   int kmmax = 0;
   for (int l = 0; l < m->L; ++l) {
     tot += (size_t)rows * m->Kx[l] * 4;
     kmmax = std::max(kmmax, m->Km[l]);
   }
-  tot += (size_t)B * C * kmmax * 4 + (size_t)B * C * m->W1 * 4;
-  if (C > 1) tot += (size_t)rows * m->W1 * 4;
+  tot += (size_t)B * Cp * kmmax * 4 + (size_t)B * Cp * m->W1 * 4;
   tot += (size_t)rows * m->W1 * 4 + (size_t)rows * m->NO * 4;
   tot += (size_t)B * m->D * C * m->n * m->n * 4;
   tot += (size_t)B * m->D * 4 * 3;
@@ -334,8 +336,9 @@ dw_status run_pass(dw_model* m, const float* r, long B, const float* R, const fl
                    float* out_lp, float* out_phase, float* out_eloc, uint32_t* out_flags) {
   dw_status st = ensure_ws(m, B, C);
   if (st) return st;
-  Shape sh{(int)B, m->n, m->u, m->I, m->D, C, m->W1, m->W2, m->L};
-  const long rows = B * m->n * C;
+  const int Cp = rows_per_electron(C);
+  Shape sh{(int)B, m->n, m->u, m->I, m->D, C, Cp, m->W1, m->W2, m->L};
+  const long rows = B * m->n * Cp;
   Timed pass_timer(m, C > 1 ? DW_TIMER_LOCAL_ENERGY : DW_TIMER_FORWARD, s);
   const GemmPrec pr = m->prec;
   std::vector<float*> X(m->L);
@@ -346,19 +349,19 @@ dw_status run_pass(dw_model* m, const float* r, long B, const float* R, const fl
     launch_spin_mean(X[l], m->Kx[l], m->Kh[l], m->MU.f(), m->Km[l], sh, s);
     GemmEpi ev;
     ev.bias = m->bias[l].f();
-    ev.C = C;
-    ev.nC = m->n * C;
+    ev.C = Cp;
+    ev.nC = m->n * Cp;
     gemm(pr, m->MU.f(), m->Km[l], m->Wmu[l].f(), m->Wmu_hi[l].h(), m->Wmu_lo[l].h(), m->Km[l], m->V.f(), m->W1,
-         (int)(B * C), m->W1, m->Km[l], ev, s);
+         (int)(B * Cp), m->W1, m->Km[l], ev, s);
     const bool last = (l == m->L - 1);
     float* outp = last ? m->H.f() : X[l + 1];
     const int ldo = last ? m->W1 : m->Kx[l + 1];
-    GemmEpi em;
-    em.V = m->V.f();
-    em.ldv = m->W1;
-    em.C = C;
-    em.nC = m->n * C;
     if (C == 1) {
+      GemmEpi em;
+      em.V = m->V.f();
+      em.ldv = m->W1;
+      em.C = 1;
+      em.nC = m->n;
       em.act_tanh = 1;
       em.permute = last ? 1 : 0;
       em.n = m->n;
@@ -368,13 +371,12 @@ dw_status run_pass(dw_model* m, const float* r, long B, const float* R, const fl
            (int)rows, m->W1, m->Kx[l], em, s);
     } else {
       Timed t(m, l >= 1 ? DW_TIMER_GEMM_1EL_LAP : -1, s);
-      gemm(pr, X[l], m->Kx[l], m->Wmain[l].f(), m->Wmain_hi[l].h(), m->Wmain_lo[l].h(), m->Kx[l], m->Zb.f(), m->W1,
-           (int)rows, m->W1, m->Kx[l], em, s);
-      launch_tanh_rule(m->Zb.f(), outp, ldo, last ? 1 : 0, sh, s);
+      gemm_lap(pr, X[l], m->Kx[l], m->Wmain[l].f(), m->Wmain_hi[l].h(), m->Wmain_lo[l].h(), m->Kx[l], m->V.f(),
+               m->W1, outp, ldo, last ? 1 : 0, sh, s);
     }
   }
   // orbital GEMMs on the spin-major rows
-  const long up_rows = B * m->u * C, dn_rows = B * m->dn * C;
+  const long up_rows = B * m->u * Cp, dn_rows = B * m->dn * Cp;
   GemmEpi e0;
   gemm(pr, m->H.f(), m->W1, m->Wo[0].f(), m->Wo_hi[0].h(), m->Wo_lo[0].h(), m->W1, m->O.f(), m->NO, (int)up_rows,
        m->NO, m->W1, e0, s);

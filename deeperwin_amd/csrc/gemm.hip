@@ -288,3 +288,288 @@ void gemm(GemmPrec prec, const float* A, int lda, const float* Bt, const uint16_
 }
 
 }  // namespace dwmi
+
+namespace dwmi {
+
+// ------------------------------------------------------------------------------------------------
+// 1-el embedding GEMM of the Laplacian pass with the tanh forward-Laplacian rule fused in the epilogue.
+// Rows are (walker, electron, channel) with Cp = pad16(3 n + 2) rows per electron; a workgroup owns E
+// whole electrons (RB = E * Cp / 16 row blocks of 16) and 128 output columns (4 waves x 32 columns),
+// so every column of every electron has all of its channels in the tile:
+//   z_ch = (X W)_ch + V[walker, ch]                     (V: spin-mean part + bias, per walker & channel)
+//   y_0 = tanh z_0,  y_k = t' z_k (k = 1..3n),  y_lap = t' z_lap + t'' sum_k z_k^2,  padded channels 0.
+// MFMA: v_mfma_f32_16x16x4_f32 (FP32) or v_mfma_f32_16x16x32_bf16 (BF16 forms, A split hi/lo in LDS).
+// ------------------------------------------------------------------------------------------------
+template <int RB, int PREC>
+__global__ __launch_bounds__(256) void k_gemm_lap(const float* __restrict__ X, int ldx,
+                                                  const float* __restrict__ Bt, const uint16_t* __restrict__ Bhi,
+                                                  const uint16_t* __restrict__ Blo, int K,
+                                                  const float* __restrict__ V, int ldv, float* __restrict__ out,
+                                                  int ldo, int permute, int E, Shape sh, int tilesN) {
+  constexpr int ROWS = RB * 16;
+  constexpr int BN = 128;
+  constexpr int BK = (PREC == 0) ? 16 : 32;
+  constexpr int LDK = BK + 8;  // bf16 row pitch (elements)
+  constexpr int EP = 64 + 4;   // epilogue pitch (floats): 64 columns per pass
+  constexpr size_t LOAD_BYTES = (PREC == 0)
+                                    ? (size_t)BK * (ROWS + 4) * 4 + (size_t)BK * (BN + 4) * 4
+                                    : (size_t)(ROWS + BN) * LDK * 2 * (PREC == 1 ? 2 : 1);
+  constexpr size_t EPI_BYTES = (size_t)ROWS * EP * 4;
+  constexpr size_t SMEM = LOAD_BYTES > EPI_BYTES ? LOAD_BYTES : EPI_BYTES;
+  __shared__ __align__(16) unsigned char smem[SMEM];
+
+  const int Cp = sh.Cp, C = sh.C, n = sh.n;
+  const long n_el_total = (long)sh.B * n;
+  const int tile_m = blockIdx.x / tilesN, tile_n = blockIdx.x % tilesN;
+  const long g0 = (long)tile_m * E;          // first electron of the tile
+  const long row0 = g0 * Cp;
+  const long rows_total = n_el_total * Cp;
+  const int col0 = tile_n * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+
+  v4f acc[RB][2];
+#pragma unroll
+  for (int a = 0; a < RB; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+
+  if constexpr (PREC == 0) {
+    float(*As)[ROWS + 4] = (float(*)[ROWS + 4])smem;
+    float(*Bs)[BN + 4] = (float(*)[BN + 4])(smem + (size_t)BK * (ROWS + 4) * 4);
+    constexpr int AV = ROWS * BK / 4;  // float4 per A tile
+    constexpr int AIT = (AV + 255) / 256;
+    float4 ra[AIT], rb[2];
+    auto gload = [&](int k0) {
+#pragma unroll
+      for (int it = 0; it < AIT; ++it) {
+        const int idx = tid + 256 * it;
+        if (idx < AV) {
+          const int m = idx >> 2, kq = (idx & 3) * 4;
+          long gr = row0 + m;
+          if (gr > rows_total - 1) gr = rows_total - 1;
+          ra[it] = *(const float4*)(X + gr * ldx + k0 + kq);
+        }
+      }
+#pragma unroll
+      for (int it = 0; it < 2; ++it) {
+        const int idx = tid + 256 * it, m = idx >> 2, kq = (idx & 3) * 4;
+        rb[it] = *(const float4*)(Bt + (long)(col0 + m) * K + k0 + kq);
+      }
+    };
+    auto sstore = [&]() {
+#pragma unroll
+      for (int it = 0; it < AIT; ++it) {
+        const int idx = tid + 256 * it;
+        if (idx < AV) {
+          const int m = idx >> 2, kq = (idx & 3) * 4;
+          As[kq + 0][m] = ra[it].x;
+          As[kq + 1][m] = ra[it].y;
+          As[kq + 2][m] = ra[it].z;
+          As[kq + 3][m] = ra[it].w;
+        }
+      }
+#pragma unroll
+      for (int it = 0; it < 2; ++it) {
+        const int idx = tid + 256 * it, m = idx >> 2, kq = (idx & 3) * 4;
+        Bs[kq + 0][m] = rb[it].x;
+        Bs[kq + 1][m] = rb[it].y;
+        Bs[kq + 2][m] = rb[it].z;
+        Bs[kq + 3][m] = rb[it].w;
+      }
+    };
+    gload(0);
+    for (int k0 = 0; k0 < K; k0 += BK) {
+      __syncthreads();
+      sstore();
+      __syncthreads();
+      if (k0 + BK < K) gload(k0 + BK);
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        const int kr = kk + (lane >> 4);
+        const float b0 = Bs[kr][wv * 32 + (lane & 15)];
+        const float b1 = Bs[kr][wv * 32 + 16 + (lane & 15)];
+#pragma unroll
+        for (int rb2 = 0; rb2 < RB; ++rb2) {
+          const float a = As[kr][rb2 * 16 + (lane & 15)];
+          acc[rb2][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, acc[rb2][0], 0, 0, 0);
+          acc[rb2][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, acc[rb2][1], 0, 0, 0);
+        }
+      }
+    }
+  } else {
+    constexpr bool X3 = (PREC == 1);
+    uint16_t(*Ah)[LDK] = (uint16_t(*)[LDK])smem;
+    uint16_t(*Bh)[LDK] = (uint16_t(*)[LDK])(smem + (size_t)ROWS * LDK * 2);
+    uint16_t(*Al)[LDK] = (uint16_t(*)[LDK])(smem + (size_t)(ROWS + BN) * LDK * 2);
+    uint16_t(*Bl)[LDK] = (uint16_t(*)[LDK])(smem + (size_t)(2 * ROWS + BN) * LDK * 2);
+    constexpr int AV = ROWS * BK / 4;
+    constexpr int AIT = (AV + 255) / 256;
+    float4 ra[AIT];
+    uint2 rbh[4], rbl[4];
+    auto gload = [&](int k0) {
+#pragma unroll
+      for (int it = 0; it < AIT; ++it) {
+        const int idx = tid + 256 * it;
+        if (idx < AV) {
+          const int m = idx >> 3, kq = (idx & 7) * 4;
+          long gr = row0 + m;
+          if (gr > rows_total - 1) gr = rows_total - 1;
+          ra[it] = *(const float4*)(X + gr * ldx + k0 + kq);
+        }
+      }
+#pragma unroll
+      for (int it = 0; it < 4; ++it) {
+        const int idx = tid + 256 * it, m = idx >> 3, kq = (idx & 7) * 4;
+        rbh[it] = *(const uint2*)(Bhi + (long)(col0 + m) * K + k0 + kq);
+        if (X3) rbl[it] = *(const uint2*)(Blo + (long)(col0 + m) * K + k0 + kq);
+      }
+    };
+    auto sstore = [&]() {
+#pragma unroll
+      for (int it = 0; it < AIT; ++it) {
+        const int idx = tid + 256 * it;
+        if (idx < AV) {
+          const int m = idx >> 3, kq = (idx & 7) * 4;
+          const float f[4] = {ra[it].x, ra[it].y, ra[it].z, ra[it].w};
+          unsigned short h[4], l[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            h[q] = f2bf_bits(f[q]);
+            l[q] = f2bf_bits(f[q] - bf_bits2f(h[q]));
+          }
+          uint2 hv, lv;
+          hv.x = h[0] | ((unsigned)h[1] << 16);
+          hv.y = h[2] | ((unsigned)h[3] << 16);
+          lv.x = l[0] | ((unsigned)l[1] << 16);
+          lv.y = l[2] | ((unsigned)l[3] << 16);
+          *(uint2*)&Ah[m][kq] = hv;
+          if (X3) *(uint2*)&Al[m][kq] = lv;
+        }
+      }
+#pragma unroll
+      for (int it = 0; it < 4; ++it) {
+        const int idx = tid + 256 * it, m = idx >> 3, kq = (idx & 7) * 4;
+        *(uint2*)&Bh[m][kq] = rbh[it];
+        if (X3) *(uint2*)&Bl[m][kq] = rbl[it];
+      }
+    };
+    gload(0);
+    for (int k0 = 0; k0 < K; k0 += BK) {
+      __syncthreads();
+      sstore();
+      __syncthreads();
+      if (k0 + BK < K) gload(k0 + BK);
+      const int kc = 8 * (lane >> 4);
+      v8s bh[2], bl[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        bh[q] = *(const v8s*)&Bh[wv * 32 + q * 16 + (lane & 15)][kc];
+        if (X3) bl[q] = *(const v8s*)&Bl[wv * 32 + q * 16 + (lane & 15)][kc];
+      }
+#pragma unroll
+      for (int rb2 = 0; rb2 < RB; ++rb2) {
+        const v8s ah = *(const v8s*)&Ah[rb2 * 16 + (lane & 15)][kc];
+        v8s al;
+        if (X3) al = *(const v8s*)&Al[rb2 * 16 + (lane & 15)][kc];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          if (X3) {
+            acc[rb2][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16((v8bf)al, (v8bf)bh[q], acc[rb2][q], 0, 0, 0);
+            acc[rb2][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16((v8bf)ah, (v8bf)bl[q], acc[rb2][q], 0, 0, 0);
+          }
+          acc[rb2][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16((v8bf)ah, (v8bf)bh[q], acc[rb2][q], 0, 0, 0);
+        }
+      }
+    }
+  }
+  // ---- epilogue: tile to LDS (two passes of 64 columns), one thread per (electron, column) applies the rule
+  float(*Es)[EP] = (float(*)[EP])smem;
+  const int cc = tid & 63;
+  const int col = col0 + (cc >> 4) * 32 + (cc & 15);  // + 16 * q below
+  const int dn = n - sh.u;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    __syncthreads();
+#pragma unroll
+    for (int rb2 = 0; rb2 < RB; ++rb2)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) Es[rb2 * 16 + 4 * (lane >> 4) + j][wv * 16 + (lane & 15)] = acc[rb2][q][j];
+    __syncthreads();
+    const int gc = col + 16 * q;
+    for (int e = tid >> 6; e < E; e += 4) {
+      const long g = g0 + e;
+      if (g >= n_el_total) break;
+      const long b = g / n;
+      const int i = (int)(g % n);
+      const float* Vb = V + (b * Cp) * (long)ldv + gc;
+      long obase;
+      if (permute)
+        obase = (i < sh.u) ? (b * sh.u + i) * Cp : (long)sh.B * sh.u * Cp + (b * dn + (i - sh.u)) * Cp;
+      else
+        obase = g * Cp;
+      const float* Ez = &Es[e * Cp][cc];
+      const float z0 = Ez[0] + Vb[0];
+      const float th = tanhf(z0);
+      const float d1 = 1.f - th * th, d2 = -2.f * th * d1;
+      out[obase * ldo + gc] = th;
+      float S = 0.f;
+      for (int ch = 1; ch < C - 1; ++ch) {
+        const float zc = Ez[ch * EP] + Vb[(long)ch * ldv];
+        S += zc * zc;
+        out[(obase + ch) * ldo + gc] = d1 * zc;
+      }
+      const float zl = Ez[(C - 1) * EP] + Vb[(long)(C - 1) * ldv];
+      out[(obase + C - 1) * ldo + gc] = d1 * zl + d2 * S;
+      for (int ch = C; ch < Cp; ++ch) out[(obase + ch) * ldo + gc] = 0.f;
+    }
+  }
+}
+
+template <int PREC>
+static void gemm_lap_rb(int RB, const float* X, int ldx, const float* Bt, const uint16_t* Bhi, const uint16_t* Blo,
+                        int K, const float* V, int ldv, float* out, int ldo, int permute, int E, const Shape& sh,
+                        hipStream_t s) {
+  const int tilesN = sh.W1 / 128;
+  const long tilesM = ((long)sh.B * sh.n + E - 1) / E;
+  const dim3 grid((unsigned)(tilesM * tilesN));
+#define DW_LAP_CASE(R)                                                                                             \
+  case R:                                                                                                          \
+    hipLaunchKernelGGL((k_gemm_lap<R, PREC>), grid, dim3(256), 0, s, X, ldx, Bt, Bhi, Blo, K, V, ldv, out, ldo, \
+                       permute, E, sh, tilesN);                                                                    \
+    break;
+  switch (RB) {
+    DW_LAP_CASE(1)
+    DW_LAP_CASE(2)
+    DW_LAP_CASE(3)
+    DW_LAP_CASE(4)
+    DW_LAP_CASE(5)
+    DW_LAP_CASE(6)
+    DW_LAP_CASE(7)
+    DW_LAP_CASE(8)
+    DW_LAP_CASE(9)
+    DW_LAP_CASE(10)
+    DW_LAP_CASE(11)
+    DW_LAP_CASE(12)
+    default: break;
+  }
+#undef DW_LAP_CASE
+}
+
+int lap_electrons_per_tile(int Cp) {
+  const int E = 96 / Cp;
+  return E < 1 ? 1 : E;
+}
+
+void gemm_lap(GemmPrec prec, const float* X, int ldx, const float* Bt, const uint16_t* Bhi, const uint16_t* Blo, int K,
+              const float* V, int ldv, float* out, int ldo, int permute, const Shape& sh, hipStream_t s) {
+  const int E = lap_electrons_per_tile(sh.Cp);
+  const int RB = E * sh.Cp / 16;
+  if (prec == GEMM_F32)
+    gemm_lap_rb<0>(RB, X, ldx, Bt, Bhi, Blo, K, V, ldv, out, ldo, permute, E, sh, s);
+  else if (prec == GEMM_BF16X3)
+    gemm_lap_rb<1>(RB, X, ldx, Bt, Bhi, Blo, K, V, ldv, out, ldo, permute, E, sh, s);
+  else
+    gemm_lap_rb<2>(RB, X, ldx, Bt, Bhi, Blo, K, V, ldv, out, ldo, permute, E, sh, s);
+}
+
+}  // namespace dwmi

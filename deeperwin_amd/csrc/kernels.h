@@ -25,15 +25,20 @@ enum GemmPrec { GEMM_F32 = 0, GEMM_BF16X3 = 1, GEMM_BF16 = 2 };
 void gemm(GemmPrec prec, const float* A, int lda, const float* Bt, const uint16_t* Bhi, const uint16_t* Blo,
           int ldb, float* C, int ldc, int M, int N, int K, const GemmEpi& ep, hipStream_t s);
 
+// B walkers, n electrons (u up), I ions, D dets, C channels per electron (1, or 3 n + 2 in the
+// Laplacian pass) stored in Cp rows (C, or pad16(C) in the Laplacian pass), widths W1 / W2, L iterations.
 struct Shape {
-  int B, n, u, I, D, C, W1, W2, L;
+  int B, n, u, I, D, C, Cp, W1, W2, L;
 };
+
+// Fused 1-el GEMM + tanh forward-Laplacian rule of the Laplacian pass (rows: Cp per electron).
+void gemm_lap(GemmPrec prec, const float* X, int ldx, const float* Bt, const uint16_t* Bhi, const uint16_t* Blo, int K,
+              const float* V, int ldv, float* out, int ldo, int permute, const Shape& sh, hipStream_t s);
 
 void launch_features0(const float* r, const float* R, float* X0, int Kx0, const Shape& sh, hipStream_t s);
 void launch_twoel(const float* r, float* const* X, const int* Kx, const int* Kh, const float* W2e, const float* b2e,
                   const Shape& sh, hipStream_t s);
 void launch_spin_mean(const float* X, int Kx, int Kh, float* MU, int Km, const Shape& sh, hipStream_t s);
-void launch_tanh_rule(const float* Z, float* out, int ldo, int permute, const Shape& sh, hipStream_t s);
 void launch_envelope(const float* O, int ldo, const float* r, const float* R, const float* kappa,
                      const float* wenv, float* A, const Shape& sh, hipStream_t s);
 void launch_det(const float* A, float* ld, float* sd, float* g, float* lapd, uint32_t* dflags, float cond_thr,
@@ -48,6 +53,5 @@ void launch_accept(float* r, const float* r_prop, float* lp, const float* lp_pro
                    int32_t* age, int max_age, int32_t* acc_count, uint32_t* flags, int B, int n, hipStream_t s);
 void launch_chain_update(void* chain, const int32_t* acc_count, int B, float* out_acc, int interval, float target,
                          float lo, float hi, hipStream_t s);
-void launch_split_bf16(const float* src, uint16_t* hi, uint16_t* lo, size_t count, hipStream_t s);
 
 }  // namespace dwmi

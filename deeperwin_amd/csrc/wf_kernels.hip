@@ -22,15 +22,19 @@ static constexpr int MAXL = 4;  // FermiNet iterations supported by the fused 2-
 // ------------------------------------------------------------------------------------------------
 __global__ void k_features0(const float* __restrict__ r, const float* __restrict__ R, float* __restrict__ X0,
                             int Kx0, Shape sh) {
-  const long total = (long)sh.B * sh.n * sh.C * Kx0;
+  const long total = (long)sh.B * sh.n * sh.Cp * Kx0;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
   const int col = (int)(t % Kx0);
   const long row = t / Kx0;
-  const int ch = (int)(row % sh.C);
-  const long bi = row / sh.C;
+  const int ch = (int)(row % sh.Cp);
+  const long bi = row / sh.Cp;
   const int i = (int)(bi % sh.n);
   const int I4 = 4 * sh.I;
+  if (ch >= sh.C) {  // padding channel rows stay zero
+    X0[t] = 0.f;
+    return;
+  }
   if (col >= I4 && col < I4 + 8) return;  // pair-mean columns: written by k_twoel
   float v = 0.f;
   if (col < I4) {
@@ -51,7 +55,7 @@ __global__ void k_features0(const float* __restrict__ r, const float* __restrict
 }
 
 void launch_features0(const float* r, const float* R, float* X0, int Kx0, const Shape& sh, hipStream_t s) {
-  const long total = (long)sh.B * sh.n * sh.C * Kx0;
+  const long total = (long)sh.B * sh.n * sh.Cp * Kx0;
   hipLaunchKernelGGL(k_features0, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, r, R, X0, Kx0, sh);
 }
 
@@ -79,7 +83,7 @@ __global__ __launch_bounds__(256) void k_twoel(const float* __restrict__ r, TwoE
                                                Shape sh) {
   constexpr int NCH = LAP ? 5 : 1;
   constexpr int NL2 = (L > 1) ? L - 1 : 1;
-  const int n = sh.n, u = sh.u, C = sh.C;
+  const int n = sh.n, u = sh.u, C = sh.C, Cp = sh.Cp;
   const int groups = (n + 7) >> 3;
   const long b = blockIdx.x / groups;
   const int il = threadIdx.x >> 5, f = threadIdx.x & 31;
@@ -145,7 +149,7 @@ __global__ __launch_bounds__(256) void k_twoel(const float* __restrict__ r, TwoE
       if (offdiag && f < 4) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          float* row = out.X[0] + (bi * C + 1 + 3 * j + c) * (long)out.Kx[0] + out.Kh[0];
+          float* row = out.X[0] + (bi * Cp + 1 + 3 * j + c) * (long)out.Kx[0] + out.Kh[0];
           row[sj * 4 + f] = x[1 + c] * inv_s;
           row[(1 - sj) * 4 + f] = 0.f;
         }
@@ -200,7 +204,7 @@ __global__ __launch_bounds__(256) void k_twoel(const float* __restrict__ r, TwoE
           const int l = l2 + 1;
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            float* row = out.X[l] + (bi * C + 1 + 3 * j + c) * (long)out.Kx[l] + out.Kh[l];
+            float* row = out.X[l] + (bi * Cp + 1 + 3 * j + c) * (long)out.Kx[l] + out.Kh[l];
             row[sj * 32 + f] = y[1 + c] * inv_s;
             row[(1 - sj) * 32 + f] = 0.f;
           }
@@ -222,11 +226,11 @@ __global__ __launch_bounds__(256) void k_twoel(const float* __restrict__ r, TwoE
       float a[NCH];
 #pragma unroll
       for (int c = 0; c < NCH; ++c) a[c] = (s == 0) ? acc0[l][c] : acc1[l][c];
-      X[(bi * C) * Kx + Kh + s * Ke + f] = a[0] * inv;
+      X[(bi * Cp) * Kx + Kh + s * Ke + f] = a[0] * inv;
       if constexpr (LAP) {
-        X[(bi * C + C - 1) * Kx + Kh + s * Ke + f] = a[4] * inv;
+        X[(bi * Cp + C - 1) * Kx + Kh + s * Ke + f] = a[4] * inv;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) X[(bi * C + 1 + 3 * i + c) * Kx + Kh + s * Ke + f] = -a[1 + c] * inv;
+        for (int c = 0; c < 3; ++c) X[(bi * Cp + 1 + 3 * i + c) * Kx + Kh + s * Ke + f] = -a[1 + c] * inv;
       }
     }
   }
@@ -261,66 +265,28 @@ void launch_twoel(const float* r, float* const* X, const int* Kx, const int* Kh,
 // Spin means of h over electrons (ferminet_embedding.py:61-62, 72): MU[b, ch] = [mean_{i<u} h, mean_{i>=u} h].
 // ------------------------------------------------------------------------------------------------
 __global__ void k_spin_mean(const float* __restrict__ X, int Kx, int Kh, float* __restrict__ MU, int Km, Shape sh) {
-  const long total = (long)sh.B * sh.C * Km;
+  const long total = (long)sh.B * sh.Cp * Km;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
   const int f = (int)(t % Km);
   const long row = t / Km;
-  const int ch = (int)(row % sh.C);
-  const long b = row / sh.C;
+  const int ch = (int)(row % sh.Cp);
+  const long b = row / sh.Cp;
   float v = 0.f;
   if (f < 2 * Kh) {
     const int s = f >= Kh ? 1 : 0;
     const int fi = f - s * Kh;
     const int i0 = s ? sh.u : 0, i1 = s ? sh.n : sh.u;
     float acc = 0.f;
-    for (int i = i0; i < i1; ++i) acc += X[((b * sh.n + i) * sh.C + ch) * (long)Kx + fi];
+    for (int i = i0; i < i1; ++i) acc += X[((b * sh.n + i) * sh.Cp + ch) * (long)Kx + fi];
     v = acc / (float)(i1 - i0);
   }
   MU[t] = v;
 }
 
 void launch_spin_mean(const float* X, int Kx, int Kh, float* MU, int Km, const Shape& sh, hipStream_t s) {
-  const long total = (long)sh.B * sh.C * Km;
+  const long total = (long)sh.B * sh.Cp * Km;
   hipLaunchKernelGGL(k_spin_mean, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, X, Kx, Kh, MU, Km, sh);
-}
-
-// ------------------------------------------------------------------------------------------------
-// tanh with forward-mode Laplacian: y = tanh(z); dy_k = t' dz_k; Lap y = t' Lap z + t'' sum_k dz_k^2.
-// One thread per (walker, electron, feature); channels are read from the pre-activation rows.
-// ------------------------------------------------------------------------------------------------
-__global__ void k_tanh_rule(const float* __restrict__ Z, float* __restrict__ out, int ldo, int permute, Shape sh) {
-  const long total = (long)sh.B * sh.n * sh.W1;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= total) return;
-  const int f = (int)(t % sh.W1);
-  const long bi = t / sh.W1;
-  const int i = (int)(bi % sh.n);
-  const long b = bi / sh.n;
-  const int C = sh.C;
-  const float* z = Z + bi * C * (long)sh.W1 + f;
-  long obase;
-  if (permute) {
-    const int dn = sh.n - sh.u;
-    obase = (i < sh.u) ? (b * sh.u + i) * C : (long)sh.B * sh.u * C + (b * dn + (i - sh.u)) * C;
-  } else {
-    obase = bi * C;
-  }
-  const float th = tanhf(z[0]);
-  const float d1 = 1.f - th * th, d2 = -2.f * th * d1;
-  out[obase * ldo + f] = th;
-  float S = 0.f;
-  for (int ch = 1; ch < C - 1; ++ch) {
-    const float zc = z[(long)ch * sh.W1];
-    S += zc * zc;
-    out[(obase + ch) * ldo + f] = d1 * zc;
-  }
-  out[(obase + C - 1) * ldo + f] = d1 * z[(long)(C - 1) * sh.W1] + d2 * S;
-}
-
-void launch_tanh_rule(const float* Z, float* out, int ldo, int permute, const Shape& sh, hipStream_t s) {
-  const long total = (long)sh.B * sh.n * sh.W1;
-  hipLaunchKernelGGL(k_tanh_rule, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, Z, out, ldo, permute, sh);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -328,7 +294,7 @@ void launch_tanh_rule(const float* Z, float* out, int ldo, int permute, const Sh
 // env_s[i, c] = sum_I w_s[I, c] exp(-kappa_s[I, c] |r_i - R_I|), kappa = softplus(alpha), with the product
 // rule for tangents (env depends on electron i only) and the Laplacian
 //   Lap A = Lap(env) bf + 2 grad_i(env) . grad_i(bf) + env Lap(bf),   Lap e^{-k rho} = (k^2 - 2k/rho) e^{-k rho}.
-// O rows are spin-major (up block then down block); A is [B, D, C, n, n].
+// O rows are spin-major (up block then down block, Cp rows per electron); A is [B, D, C, n, n].
 // ------------------------------------------------------------------------------------------------
 __global__ void k_envelope(const float* __restrict__ O, int ldo, const float* __restrict__ r,
                            const float* __restrict__ R, const float* __restrict__ kappa,
@@ -343,7 +309,8 @@ __global__ void k_envelope(const float* __restrict__ O, int ldo, const float* __
   const long b = bi / n;
   const int s = (i < sh.u) ? 0 : 1;
   const int dn = n - sh.u;
-  const long orow = (s == 0) ? (b * sh.u + i) * C : (long)sh.B * sh.u * C + (b * dn + (i - sh.u)) * C;
+  const int Cp = sh.Cp;
+  const long orow = (s == 0) ? (b * sh.u + i) * Cp : (long)sh.B * sh.u * Cp + (b * dn + (i - sh.u)) * Cp;
   const float* ri = r + bi * 3;
   const float* kap = kappa + (long)s * sh.I * NO;
   const float* we = wenv + (long)s * sh.I * NO;

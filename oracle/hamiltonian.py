@@ -3,8 +3,10 @@
 * ``potential_energy``  <- get_potential_energy and helpers (src/deeperwin/hamiltonian.py:17-39)
 * ``kinetic_energy``    <- get_kinetic_energy, default forward_lap=False branch (hamiltonian.py:234-266):
   g = grad(log psi^2); the Hessian diagonal comes from 3n Jacobian-vector products of g with unit vectors
-  (the reference uses jax.linearize + fori_loop; torch.func.jvp under vmap computes the same products);
-  E_kin = -1/2 (1/4 |g|^2 + 1/2 tr H).
+  (the reference uses jax.linearize + fori_loop; torch.func.jvp under vmap over the 3n tangents computes
+  the same products); E_kin = -1/2 (1/4 |g|^2 + 1/2 tr H).  Walkers are looped over one by one: in this
+  PyTorch build, vmap over walkers of jvp(grad(slogdet(M))) returns wrong values for batch elements > 0
+  when each walker has a single determinant (an un-batched matrix), so the walker loop stays explicit.
 * ``local_energy``      <- build_local_energy (hamiltonian.py:272-291), walkers processed in chunks of
   ``max_batch_size`` = 64 like folx.batched_vmap.
 """
@@ -56,7 +58,8 @@ def kinetic_energy(params: Dict, n_up: int, n_dn: int, r: torch.Tensor, R: torch
         lap = torch.diagonal(hess_cols).sum()
         return -0.5 * (0.25 * (g ** 2).sum() + 0.5 * lap)
 
-    return torch.func.vmap(one_walker)(r.reshape(r.shape[0], 3 * n).to(dtype))
+    x = r.reshape(r.shape[0], 3 * n).to(dtype)
+    return torch.stack([one_walker(x[b]) for b in range(x.shape[0])])
 
 
 def local_energy(params: Dict, n_up: int, n_dn: int, r, R, Z, max_batch_size: int = 64,

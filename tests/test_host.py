@@ -1,0 +1,114 @@
+"""Host-side logic (CPU only): parameter layout and initialisation, suffix matching, per-geometry
+truncation, electron-to-ion mapping, seeded geometries and walker initialisation."""
+import numpy as np
+import pytest
+
+from deeperwin_amd.config import MCMCConfig, ModelConfigFermiNet, PhysicalConfig
+from deeperwin_amd.params import ENV, ModelShape, find_leaf, infer_shape, init_params, truncate_params
+from deeperwin_amd.physics import (batch_geometries, generate_el_ion_mapping, geometry_c4h2, init_walkers_gaussian,
+                                   random_molecule)
+
+
+def test_param_shapes_and_init_bounds():
+    s = ModelShape(14, 7, 2, 16)
+    p = init_params(s, 1)
+    w0 = find_leaf(p, "h_el_0/linear_0", "w")
+    assert w0.shape == (12 * 2 + 8, 256)
+    assert find_leaf(p, "h_el_1/linear_0", "w").shape == (3 * 256 + 64, 256)
+    assert find_leaf(p, "h_el_el_0/linear_0", "w").shape == (4, 32)
+    assert find_leaf(p, "h_el_el_2/linear_0", "w").shape == (32, 32)
+    assert find_leaf(p, "bf_up/linear_0", "w").shape == (256, 16 * 14)
+    assert find_leaf(p, "envelope_orbitals", "alpha_dn").shape == (2, 16 * 14)
+    # VarianceScaling(1, fan_avg, uniform): |w| <= sqrt(3 / fan_avg); biases 0; envelopes 1
+    lim = np.sqrt(3.0 / ((32 + 256) / 2))
+    assert np.abs(w0).max() <= lim and np.abs(w0).max() > 0.9 * lim
+    assert np.all(find_leaf(p, "h_el_2/linear_0", "b") == 0)
+    assert np.all(p[ENV]["weights_up"] == 1)
+    assert infer_shape(p, 14, 7) == s
+
+
+def test_suffix_matching_accepts_tilde_paths():
+    p = init_params(ModelShape(4, 2, 2, 2), 0)
+    q = {k.replace("wf/", "wf/~/"): v for k, v in p.items()}
+    np.testing.assert_array_equal(find_leaf(q, "h_el_3/linear_0", "w"), find_leaf(p, "h_el_3/linear_0", "w"))
+    with pytest.raises(KeyError):
+        find_leaf(p, "h_el_9/linear_0", "w")
+
+
+def test_truncation_selects_ion_and_orbital_slices():
+    big = ModelShape(10, 5, 4, 3)
+    p = init_params(big, 2)
+    small = ModelShape(6, 3, 2, 3)
+    t = truncate_params(p, n_ion_max=4, n_el_max=10, shape=small)
+    w0b, w0s = find_leaf(p, "h_el_0/linear_0", "w"), find_leaf(t, "h_el_0/linear_0", "w")
+    assert w0s.shape == (3 * 8 + 8, 256)
+    np.testing.assert_array_equal(w0s[:8], w0b[:8])            # h block, ions 0..1
+    np.testing.assert_array_equal(w0s[8:16], w0b[16:24])       # mean-up block
+    np.testing.assert_array_equal(w0s[16:24], w0b[32:40])      # mean-down block
+    np.testing.assert_array_equal(w0s[24:], w0b[48:])          # pair means
+    wb, ws = find_leaf(p, "bf_up/linear_0", "w"), find_leaf(t, "bf_up/linear_0", "w")
+    assert ws.shape == (256, 3 * 6)
+    np.testing.assert_array_equal(ws[:, 6:12], wb[:, 10:16])   # det 1, orbitals 0..5
+    assert find_leaf(t, "envelope_orbitals", "alpha_up").shape == (2, 18)
+    assert infer_shape(t, 6, 3) == small
+
+
+@pytest.mark.parametrize("R,Z,n_up", [([[0, 0, 0], [3, 0, 0]], [3, 1], 2),
+                                      ([[0, 0, 0], [2.07, 0, 0]], [7, 7], 7)])
+def test_el_ion_mapping_fills_every_ion(R, Z, n_up):
+    n = sum(Z)
+    m = generate_el_ion_mapping(R, Z, n, n_up)
+    assert len(m) == n
+    assert np.bincount(m, minlength=len(Z)).tolist() == list(Z)
+    # spin balance per ion stays within one electron for the homonuclear dimer
+    if Z[0] == Z[1]:
+        up = np.bincount(m[:n_up], minlength=2)
+        assert abs(up[0] - up[1]) <= 1
+
+
+def test_physical_config_defaults():
+    pc = PhysicalConfig(R=[[0, 0, 0], [2.07, 0, 0]], Z=[7, 7])
+    assert pc.n_electrons == 14 and pc.n_up == 7 and len(pc.el_ion_mapping) == 14
+    pc = PhysicalConfig(R=[[0, 0, 0]], Z=[5])
+    assert pc.n_up == 3  # odd electron count: the extra electron is spin up
+
+
+def test_geometries():
+    R, Z = geometry_c4h2(3)
+    assert Z.tolist() == [6, 6, 6, 6, 1, 1] and Z.sum() == 26
+    d = np.linalg.norm(R[:4, None] - R[None, :4], axis=-1)
+    sides = sorted(d[np.triu_indices(4, 1)])[:4]
+    np.testing.assert_allclose(sides, 2.6, atol=1e-9)
+    np.testing.assert_allclose(np.linalg.norm(R[4] - R[0]), 2.0, atol=1e-9)
+    rng = np.random.default_rng(0)
+    for n in (4, 17, 42):
+        R, Z = random_molecule(rng, n)
+        assert Z.sum() == n and set(Z.tolist()) <= {1, 6, 7, 8}
+        if len(Z) > 1:
+            dd = np.linalg.norm(R[:, None] - R[None], axis=-1) + np.eye(len(Z)) * 99
+            nn = dd.min(axis=1)
+            assert nn.min() >= 1.8 - 1e-9 and nn.max() <= 2.8 + 1e-9
+    geos = batch_geometries()
+    assert len(geos) == 64
+    assert all(4 <= Z.sum() <= 42 for _, Z in geos)
+
+
+def test_gaussian_walkers_centered_on_mapped_nuclei():
+    R = np.array([[0.0, 0, 0], [3.0, 0, 0]])
+    m = [0, 0, 0, 1]
+    r = init_walkers_gaussian(np.random.default_rng(0), 20000, R, m)
+    np.testing.assert_allclose(r.mean(0), R[m], atol=0.05)
+    np.testing.assert_allclose(r.std(0), 1.0, atol=0.05)
+
+
+def test_config_validation():
+    ModelConfigFermiNet().validate()
+    MCMCConfig().validate()
+    c = ModelConfigFermiNet()
+    c.orbitals.determinant_schema = "block_diag"
+    with pytest.raises(NotImplementedError):
+        c.validate()
+    m = MCMCConfig()
+    m.proposal.name = "langevin"
+    with pytest.raises(NotImplementedError):
+        m.validate()
