@@ -17,8 +17,8 @@ from .params import ModelShape
 
 LIB_PATH = Path(os.environ.get("DWMI_LIB", Path(__file__).resolve().parent / "libdwmi.so"))
 
-DW_FP32, DW_BF16X3, DW_BF16 = 0, 1, 2
-PRECISIONS = {"fp32": DW_FP32, "bf16x3": DW_BF16X3, "bf16": DW_BF16}
+DW_FP32, DW_BF16X3, DW_BF16, DW_FP64 = 0, 1, 2, 3
+PRECISIONS = {"fp32": DW_FP32, "bf16x3": DW_BF16X3, "bf16": DW_BF16, "fp64": DW_FP64}
 FLAG_NONFINITE, FLAG_FP64_DET, FLAG_FORCED_ACCEPT = 1, 2, 4
 TIMER_GEMM_1EL_LAP, TIMER_DET_LAP, TIMER_FORWARD, TIMER_LOCAL_ENERGY = 0, 1, 2, 3
 
@@ -44,6 +44,7 @@ SIGNATURES = {
                        _vp, _vp], ctypes.c_int),
     "dw_workspace_size": ([_vp, _i32, _i32], ctypes.c_size_t),
     "dw_model_set_chunk": ([_vp, _i32], ctypes.c_int),
+    "dw_model_conditioning": ([_vp, _vp, _i32, _vp], ctypes.c_int),
     "dw_model_set_timing": ([_vp, _i32], ctypes.c_int),
     "dw_model_timer_query": ([_vp, _i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)],
                              ctypes.c_int),
@@ -143,6 +144,14 @@ class NativeModel:
         cnt = ctypes.c_int32()
         _check(self.lib.dw_model_timer_query(self.handle, int(kind), ctypes.byref(ms), ctypes.byref(cnt)))
         return ms.value, cnt.value
+
+    def conditioning(self, n_walkers: int):
+        """Per-walker conditioning estimate of the last pass, as a torch CUDA tensor."""
+        import torch
+
+        out = torch.empty(n_walkers, device="cuda", dtype=torch.float32)
+        _check(self.lib.dw_model_conditioning(self.handle, _ptr(out), int(n_walkers), _stream()))
+        return out
 
     def workspace_bytes(self, n_walkers: int, mode: int) -> int:
         return int(self.lib.dw_workspace_size(self.handle, int(n_walkers), int(mode)))

@@ -1,7 +1,10 @@
 // C ABI of the dwmi library (declared in include/dwmi.h): parameter ingestion, device weight layouts,
-// workspace management and the three drivers (log|psi|, local energy, MCMC).
+// workspace management and the three drivers (log|psi|, local energy, MCMC).  The wavefunction pass is a
+// template over the activation type: float (FP32 / BF16 GEMM forms) or double (FP64 form and the FP64
+// recompute of badly conditioned walkers).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -49,10 +52,10 @@ struct DevBuf {
     p = nullptr;
     bytes = 0;
   }
-  float* f() const { return (float*)p; }
-  uint16_t* h() const { return (uint16_t*)p; }
-  int32_t* i() const { return (int32_t*)p; }
-  uint32_t* u() const { return (uint32_t*)p; }
+  template <typename T>
+  T* as() const {
+    return (T*)p;
+  }
 };
 
 struct Leaf {
@@ -97,6 +100,32 @@ bool ends_with(const std::string& s, const std::string& suf) {
          (s.size() == suf.size() || s[s.size() - suf.size() - 1] == '/');
 }
 
+int pad_to(int x, int mult) { return (x + mult - 1) / mult * mult; }
+int rows_per_electron(int C) { return C > 1 ? pad_to(C, 16) : 1; }
+
+// Device weight layouts for one activation type.
+struct Weights {
+  bool ready = false;
+  std::vector<DevBuf> Wmain, Wmu, bias, Wmain_hi, Wmain_lo, Wmu_hi, Wmu_lo;
+  DevBuf W2e, b2e, Wo[2], Wo_hi[2], Wo_lo[2], kappa, wenv;
+  void release() {
+    for (auto* v : {&Wmain, &Wmu, &bias, &Wmain_hi, &Wmain_lo, &Wmu_hi, &Wmu_lo})
+      for (auto& b : *v) b.release();
+    for (DevBuf* b : {&W2e, &b2e, &Wo[0], &Wo[1], &Wo_hi[0], &Wo_hi[1], &Wo_lo[0], &Wo_lo[1], &kappa, &wenv}) b->release();
+    ready = false;
+  }
+};
+
+// Activation workspace for one activation type.
+struct Workspace {
+  std::vector<DevBuf> X;
+  DevBuf MU, V, Z, H, O, A;
+  void release() {
+    for (auto& b : X) b.release();
+    for (DevBuf* b : {&MU, &V, &Z, &H, &O, &A}) b->release();
+  }
+};
+
 }  // namespace
 
 struct dw_model {
@@ -104,16 +133,15 @@ struct dw_model {
   int n = 0, u = 0, dn = 0, I = 0, D = 0, W1 = 0, W2 = 0, L = 0, NO = 0, NOp = 0;
   std::vector<int> Kh, Ke, Kx, Km;
   std::map<std::string, Leaf> leaves;  // key "<module>:<leaf>"
+  std::vector<std::pair<std::string, std::vector<int64_t>>> expected;
   bool dirty = true;
   int chunk = 0;
   GemmPrec prec = GEMM_F32;
-  // device weights
-  std::vector<DevBuf> Wmain, Wmu, bias, Wmain_hi, Wmain_lo, Wmu_hi, Wmu_lo;
-  DevBuf W2e, b2e, Wo[2], Wo_hi[2], Wo_lo[2], kappa, wenv;
-  // workspace
-  std::vector<DevBuf> X;
-  DevBuf MU, V, H, O, A, ldd, sdd, gd, lapd, dfl, rprop, lpprop, cnt;
-  std::vector<std::pair<std::string, std::vector<int64_t>>> expected;
+  Weights wf, wd;    // float and double layouts
+  Workspace sf, sd;  // float and double activations
+  DevBuf ldd, sdd, gd, lapd, dcond, kappa_w;
+  DevBuf rprop, lpprop, cnt;
+  DevBuf sub_r, sub_e, sub_lp, sub_idx, own_flags;  // FP64 recompute of flagged walkers
   // measurement hooks
   bool timing = false;
   struct TimerRec {
@@ -129,17 +157,17 @@ struct dw_model {
       (void)hipEventDestroy(t.b);
     }
     for (auto e : event_pool) (void)hipEventDestroy(e);
-    for (auto* v : {&Wmain, &Wmu, &bias, &Wmain_hi, &Wmain_lo, &Wmu_hi, &Wmu_lo, &X})
-      for (auto& b : *v) b.release();
-    for (DevBuf* b : {&W2e, &b2e, &Wo[0], &Wo[1], &Wo_hi[0], &Wo_hi[1], &Wo_lo[0], &Wo_lo[1], &kappa, &wenv, &MU, &V,
-                      &H, &O, &A, &ldd, &sdd, &gd, &lapd, &dfl, &rprop, &lpprop, &cnt})
+    wf.release();
+    wd.release();
+    sf.release();
+    sd.release();
+    for (DevBuf* b : {&ldd, &sdd, &gd, &lapd, &dcond, &kappa_w, &rprop, &lpprop, &cnt, &sub_r, &sub_e, &sub_lp,
+                      &sub_idx, &own_flags})
       b->release();
   }
 };
 
 namespace {
-
-int pad_to(int x, int mult) { return (x + mult - 1) / mult * mult; }
 
 std::string leaf_key(const std::string& module, const std::string& leaf) { return module + ":" + leaf; }
 
@@ -169,6 +197,12 @@ dw_status upload(DevBuf& b, const std::vector<T>& host) {
   return DW_OK;
 }
 
+template <typename T>
+dw_status upload_as(DevBuf& b, const std::vector<float>& host) {
+  std::vector<T> v(host.begin(), host.end());
+  return upload(b, v);
+}
+
 dw_status upload_split(DevBuf& hi, DevBuf& lo, const std::vector<float>& w) {
   std::vector<uint16_t> h(w.size()), l(w.size());
   for (size_t k = 0; k < w.size(); ++k) {
@@ -180,19 +214,21 @@ dw_status upload_split(DevBuf& hi, DevBuf& lo, const std::vector<float>& w) {
   return upload(lo, l);
 }
 
-// Build the device layouts of all weights from the host leaves.
-dw_status prepare(dw_model* m) {
-  if (!m->dirty) return DW_OK;
+// Build the device layouts of all weights from the host leaves, in float or double.
+dw_status prepare_weights(dw_model* m, Weights& W, bool dbl) {
   for (auto& e : m->expected)
     if (!m->leaves[e.first].set) return fail(DW_ERR_STATE, "parameter not set: " + e.first);
   const int W1 = m->W1, W2 = m->W2;
-  const bool split = m->prec != GEMM_F32;
+  const bool split = !dbl && m->prec != GEMM_F32;
+  for (auto* v : {&W.Wmain, &W.Wmu, &W.bias, &W.Wmain_hi, &W.Wmain_lo, &W.Wmu_hi, &W.Wmu_lo}) v->resize(m->L);
+  auto up = [&](DevBuf& b, const std::vector<float>& v) { return dbl ? upload_as<double>(b, v) : upload(b, v); };
   for (int l = 0; l < m->L; ++l) {
     char buf[64];
     std::snprintf(buf, sizeof buf, "h_el_%d/linear_0", l);
     const Leaf& w = m->leaves[leaf_key(buf, "w")];
     const Leaf& b = m->leaves[leaf_key(buf, "b")];
     const int Kh = m->Kh[l], Ke = m->Ke[l], Kx = m->Kx[l], Km = m->Km[l];
+    // reference row blocks of W: [h (Kh) | mean_up h (Kh) | mean_dn h (Kh) | pair up (Ke) | pair dn (Ke)]
     std::vector<float> wm((size_t)W1 * Kx, 0.f), wu((size_t)W1 * Km, 0.f);
     for (int o = 0; o < W1; ++o) {
       for (int k = 0; k < Kh; ++k) wm[(size_t)o * Kx + k] = w.data[(size_t)k * W1 + o];
@@ -200,12 +236,12 @@ dw_status prepare(dw_model* m) {
       for (int k = 0; k < 2 * Kh; ++k) wu[(size_t)o * Km + k] = w.data[(size_t)(Kh + k) * W1 + o];
     }
     dw_status st;
-    if ((st = upload(m->Wmain[l], wm))) return st;
-    if ((st = upload(m->Wmu[l], wu))) return st;
-    if ((st = upload(m->bias[l], b.data))) return st;
+    if ((st = up(W.Wmain[l], wm))) return st;
+    if ((st = up(W.Wmu[l], wu))) return st;
+    if ((st = up(W.bias[l], b.data))) return st;
     if (split) {
-      if ((st = upload_split(m->Wmain_hi[l], m->Wmain_lo[l], wm))) return st;
-      if ((st = upload_split(m->Wmu_hi[l], m->Wmu_lo[l], wu))) return st;
+      if ((st = upload_split(W.Wmain_hi[l], W.Wmain_lo[l], wm))) return st;
+      if ((st = upload_split(W.Wmu_hi[l], W.Wmu_lo[l], wu))) return st;
     }
   }
   {
@@ -222,80 +258,106 @@ dw_status prepare(dw_model* m) {
       for (int f = 0; f < W2; ++f) b2[(size_t)l * 32 + f] = b.data[f];
     }
     dw_status st;
-    if ((st = upload(m->W2e, w2))) return st;
-    if ((st = upload(m->b2e, b2))) return st;
+    if ((st = up(W.W2e, w2))) return st;
+    if ((st = up(W.b2e, b2))) return st;
   }
   const char* bfk[2] = {"bf_up/linear_0", "bf_dn/linear_0"};
   const char* ak[2] = {"alpha_up", "alpha_dn"};
   const char* wk[2] = {"weights_up", "weights_dn"};
-  std::vector<float> kap((size_t)2 * m->I * m->NO), wen((size_t)2 * m->I * m->NO);
+  std::vector<double> kap((size_t)2 * m->I * m->NO), wen((size_t)2 * m->I * m->NO);
   for (int s = 0; s < 2; ++s) {
     const Leaf& w = m->leaves[leaf_key(bfk[s], "w")];
     std::vector<float> wo((size_t)m->NOp * W1, 0.f);
     for (int c = 0; c < m->NO; ++c)
       for (int k = 0; k < W1; ++k) wo[(size_t)c * W1 + k] = w.data[(size_t)k * m->NO + c];
     dw_status st;
-    if ((st = upload(m->Wo[s], wo))) return st;
-    if (split && (st = upload_split(m->Wo_hi[s], m->Wo_lo[s], wo))) return st;
+    if ((st = up(W.Wo[s], wo))) return st;
+    if (split && (st = upload_split(W.Wo_hi[s], W.Wo_lo[s], wo))) return st;
     const Leaf& a = m->leaves[leaf_key("envelope_orbitals", ak[s])];
     const Leaf& we = m->leaves[leaf_key("envelope_orbitals", wk[s])];
     for (size_t q = 0; q < (size_t)m->I * m->NO; ++q) {
       const double x = a.data[q];
-      const double sp = x > 30.0 ? x : std::log1p(std::exp(x));  // softplus
-      kap[(size_t)s * m->I * m->NO + q] = (float)sp;
+      kap[(size_t)s * m->I * m->NO + q] = x > 30.0 ? x : std::log1p(std::exp(x));  // softplus
       wen[(size_t)s * m->I * m->NO + q] = we.data[q];
     }
   }
   dw_status st;
-  if ((st = upload(m->kappa, kap))) return st;
-  if ((st = upload(m->wenv, wen))) return st;
+  if (dbl) {
+    if ((st = upload(W.kappa, kap))) return st;
+    if ((st = upload(W.wenv, wen))) return st;
+  } else {
+    std::vector<float> kf(kap.begin(), kap.end()), wf(wen.begin(), wen.end());
+    if ((st = upload(W.kappa, kf))) return st;
+    if ((st = upload(W.wenv, wf))) return st;
+  }
   HIP_TRY(hipDeviceSynchronize());
-  m->dirty = false;
+  W.ready = true;
   return DW_OK;
 }
 
-int rows_per_electron(int C) { return C > 1 ? pad_to(C, 16) : 1; }
-
-// Size the workspace for B walkers with C channels.
-dw_status ensure_ws(dw_model* m, long B, int C) {
-  const int Cp = rows_per_electron(C);
-  const long rows = B * m->n * Cp;
-  for (int l = 0; l < m->L; ++l) HIP_TRY(m->X[l].ensure((size_t)rows * m->Kx[l] * 4));
-  int kmmax = 0;
-  for (int l = 0; l < m->L; ++l) kmmax = std::max(kmmax, m->Km[l]);
-  HIP_TRY(m->MU.ensure((size_t)B * Cp * kmmax * 4));
-  HIP_TRY(m->V.ensure((size_t)B * Cp * m->W1 * 4));
-  HIP_TRY(m->H.ensure((size_t)rows * m->W1 * 4));
-  HIP_TRY(m->O.ensure((size_t)rows * m->NO * 4));
-  HIP_TRY(m->A.ensure((size_t)B * m->D * C * m->n * m->n * 4));
-  HIP_TRY(m->ldd.ensure((size_t)B * m->D * 4));
-  HIP_TRY(m->sdd.ensure((size_t)B * m->D * 4));
-  HIP_TRY(m->dfl.ensure((size_t)B * m->D * 4));
-  if (C > 1) {
-    HIP_TRY(m->gd.ensure((size_t)B * m->D * 3 * m->n * 4));
-    HIP_TRY(m->lapd.ensure((size_t)B * m->D * 4));
-  } else {
-    HIP_TRY(m->gd.ensure(4));
-    HIP_TRY(m->lapd.ensure(4));
+dw_status prepare(dw_model* m, bool need_double) {
+  if (m->dirty) {
+    m->wf.ready = false;
+    m->wd.ready = false;
+    m->dirty = false;
+  }
+  const bool need_float = m->prec != GEMM_F64;
+  if (need_float && !m->wf.ready) {
+    dw_status st = prepare_weights(m, m->wf, false);
+    if (st) {
+      m->dirty = true;
+      return st;
+    }
+  }
+  if ((need_double || !need_float) && !m->wd.ready) {
+    dw_status st = prepare_weights(m, m->wd, true);
+    if (st) {
+      m->dirty = true;
+      return st;
+    }
   }
   return DW_OK;
 }
 
-size_t ws_bytes(const dw_model* m, long B, int C) {
+size_t ws_bytes(const dw_model* m, long B, int C, size_t esz) {
   const int Cp = rows_per_electron(C);
   const long rows = B * m->n * Cp;
   size_t tot = 0;
   int kmmax = 0;
   for (int l = 0; l < m->L; ++l) {
-    tot += (size_t)rows * m->Kx[l] * 4;
+    tot += (size_t)rows * m->Kx[l] * esz;
     kmmax = std::max(kmmax, m->Km[l]);
   }
-  tot += (size_t)B * Cp * kmmax * 4 + (size_t)B * Cp * m->W1 * 4;
-  tot += (size_t)rows * m->W1 * 4 + (size_t)rows * m->NO * 4;
-  tot += (size_t)B * m->D * C * m->n * m->n * 4;
-  tot += (size_t)B * m->D * 4 * 3;
-  if (C > 1) tot += (size_t)B * m->D * (3 * m->n + 1) * 4;
+  tot += (size_t)B * Cp * kmmax * esz + (size_t)B * Cp * m->W1 * esz;
+  if (esz == 8 && C > 1) tot += (size_t)rows * m->W1 * esz;  // unfused pre-activations (FP64 form)
+  tot += (size_t)rows * m->W1 * esz + (size_t)rows * m->NO * esz;
+  tot += (size_t)B * m->D * C * m->n * m->n * esz;
+  tot += (size_t)B * m->D * (8 * 2 + 4);
+  if (C > 1) tot += (size_t)B * m->D * (3 * m->n + 1) * 8;
   return tot;
+}
+
+// Size the workspace for B walkers with C channels and activation size esz.
+dw_status ensure_ws(dw_model* m, Workspace& w, long B, int C, size_t esz) {
+  const int Cp = rows_per_electron(C);
+  const long rows = B * m->n * Cp;
+  w.X.resize(m->L);
+  for (int l = 0; l < m->L; ++l) HIP_TRY(w.X[l].ensure((size_t)rows * m->Kx[l] * esz));
+  int kmmax = 0;
+  for (int l = 0; l < m->L; ++l) kmmax = std::max(kmmax, m->Km[l]);
+  HIP_TRY(w.MU.ensure((size_t)B * Cp * kmmax * esz));
+  HIP_TRY(w.V.ensure((size_t)B * Cp * m->W1 * esz));
+  if (esz == 8 && C > 1) HIP_TRY(w.Z.ensure((size_t)rows * m->W1 * esz));
+  HIP_TRY(w.H.ensure((size_t)rows * m->W1 * esz));
+  HIP_TRY(w.O.ensure((size_t)rows * m->NO * esz));
+  HIP_TRY(w.A.ensure((size_t)B * m->D * C * m->n * m->n * esz));
+  HIP_TRY(m->ldd.ensure((size_t)B * m->D * 8));
+  HIP_TRY(m->sdd.ensure((size_t)B * m->D * 8));
+  HIP_TRY(m->dcond.ensure((size_t)B * m->D * 4));
+  HIP_TRY(m->gd.ensure((size_t)std::max<long>(1, C > 1 ? B * m->D * 3 * m->n : 1) * 8));
+  HIP_TRY(m->lapd.ensure((size_t)std::max<long>(1, C > 1 ? B * m->D : 1) * 8));
+  HIP_TRY(m->kappa_w.ensure((size_t)B * 4));
+  return DW_OK;
 }
 
 hipEvent_t get_event(dw_model* m) {
@@ -309,13 +371,12 @@ hipEvent_t get_event(dw_model* m) {
   return e;
 }
 
-// Bracket a region with events on stream s when timing is on.
+// Bracket a region with events on stream s when timing is on (kind < 0: not timed).
 struct Timed {
   dw_model* m;
   int kind;
   hipStream_t s;
   hipEvent_t a = nullptr;
-  // kind < 0: not timed
   Timed(dw_model* m_, int kind_, hipStream_t s_) : m(m_), kind(kind_), s(s_) {
     if (m->timing && kind >= 0) {
       a = get_event(m);
@@ -331,66 +392,93 @@ struct Timed {
   }
 };
 
+// ---- per-type GEMM front ends ----
+void gemm_t(const dw_model* m, const float* A, int lda, const float* Bt, const DevBuf& hi, const DevBuf& lo, int ldb,
+            float* C, int ldc, long M, int N, int K, const GemmEpi& ep, hipStream_t s) {
+  gemm(m->prec, A, lda, Bt, hi.as<uint16_t>(), lo.as<uint16_t>(), ldb, C, ldc, (int)M, N, K, ep, s);
+}
+void gemm_t(const dw_model*, const double* A, int lda, const double* Bt, const DevBuf&, const DevBuf&, int ldb,
+            double* C, int ldc, long M, int N, int K, const GemmEpiD& ep, hipStream_t s) {
+  gemm_f64(A, lda, Bt, ldb, C, ldc, (int)M, N, K, ep, s);
+}
+
 // One pass of the wavefunction over B walkers with C channels (C == 1: value; C == 3n+2: Laplacian).
+template <typename T>
 dw_status run_pass(dw_model* m, const float* r, long B, const float* R, const float* Z, int C, hipStream_t s,
-                   float* out_lp, float* out_phase, float* out_eloc, uint32_t* out_flags) {
-  dw_status st = ensure_ws(m, B, C);
+                   float* out_lp, float* out_phase, float* out_eloc, uint32_t* out_flags, float cond_thr) {
+  constexpr bool DBL = sizeof(T) == 8;
+  Weights& W = DBL ? m->wd : m->wf;
+  Workspace& ws = DBL ? m->sd : m->sf;
+  dw_status st = ensure_ws(m, ws, B, C, sizeof(T));
   if (st) return st;
   const int Cp = rows_per_electron(C);
   Shape sh{(int)B, m->n, m->u, m->I, m->D, C, Cp, m->W1, m->W2, m->L};
   const long rows = B * m->n * Cp;
   Timed pass_timer(m, C > 1 ? DW_TIMER_LOCAL_ENERGY : DW_TIMER_FORWARD, s);
-  const GemmPrec pr = m->prec;
-  std::vector<float*> X(m->L);
-  for (int l = 0; l < m->L; ++l) X[l] = m->X[l].f();
-  launch_features0(r, R, X[0], m->Kx[0], sh, s);
-  launch_twoel(r, X.data(), m->Kx.data(), m->Kh.data(), m->W2e.f(), m->b2e.f(), sh, s);
+  std::vector<T*> X(m->L);
+  for (int l = 0; l < m->L; ++l) X[l] = ws.X[l].as<T>();
+  launch_features0<T>(r, R, X[0], m->Kx[0], sh, s);
+  launch_twoel<T>(r, X.data(), m->Kx.data(), m->Kh.data(), W.W2e.as<T>(), W.b2e.as<T>(), sh, s);
   for (int l = 0; l < m->L; ++l) {
-    launch_spin_mean(X[l], m->Kx[l], m->Kh[l], m->MU.f(), m->Km[l], sh, s);
-    GemmEpi ev;
-    ev.bias = m->bias[l].f();
+    launch_spin_mean<T>(X[l], m->Kx[l], m->Kh[l], ws.MU.as<T>(), m->Km[l], sh, s);
+    GemmEpiT<T> ev;
+    ev.bias = W.bias[l].as<T>();
     ev.C = Cp;
     ev.nC = m->n * Cp;
-    gemm(pr, m->MU.f(), m->Km[l], m->Wmu[l].f(), m->Wmu_hi[l].h(), m->Wmu_lo[l].h(), m->Km[l], m->V.f(), m->W1,
-         (int)(B * Cp), m->W1, m->Km[l], ev, s);
+    gemm_t(m, ws.MU.as<T>(), m->Km[l], W.Wmu[l].as<T>(), W.Wmu_hi[l], W.Wmu_lo[l], m->Km[l], ws.V.as<T>(), m->W1,
+           B * Cp, m->W1, m->Km[l], ev, s);
     const bool last = (l == m->L - 1);
-    float* outp = last ? m->H.f() : X[l + 1];
+    T* outp = last ? ws.H.as<T>() : X[l + 1];
     const int ldo = last ? m->W1 : m->Kx[l + 1];
+    GemmEpiT<T> em;
+    em.V = ws.V.as<T>();
+    em.ldv = m->W1;
+    em.C = Cp;
+    em.nC = m->n * Cp;
+    em.n = m->n;
+    em.u = m->u;
+    em.B = (int)B;
     if (C == 1) {
-      GemmEpi em;
-      em.V = m->V.f();
-      em.ldv = m->W1;
-      em.C = 1;
-      em.nC = m->n;
       em.act_tanh = 1;
       em.permute = last ? 1 : 0;
-      em.n = m->n;
-      em.u = m->u;
-      em.B = (int)B;
-      gemm(pr, X[l], m->Kx[l], m->Wmain[l].f(), m->Wmain_hi[l].h(), m->Wmain_lo[l].h(), m->Kx[l], outp, ldo,
-           (int)rows, m->W1, m->Kx[l], em, s);
+      gemm_t(m, X[l], m->Kx[l], W.Wmain[l].as<T>(), W.Wmain_hi[l], W.Wmain_lo[l], m->Kx[l], outp, ldo, rows, m->W1,
+             m->Kx[l], em, s);
     } else {
       Timed t(m, l >= 1 ? DW_TIMER_GEMM_1EL_LAP : -1, s);
-      gemm_lap(pr, X[l], m->Kx[l], m->Wmain[l].f(), m->Wmain_hi[l].h(), m->Wmain_lo[l].h(), m->Kx[l], m->V.f(),
-               m->W1, outp, ldo, last ? 1 : 0, sh, s);
+      if constexpr (DBL) {
+        gemm_t(m, X[l], m->Kx[l], W.Wmain[l].as<T>(), W.Wmain_hi[l], W.Wmain_lo[l], m->Kx[l], ws.Z.as<T>(), m->W1,
+               rows, m->W1, m->Kx[l], em, s);
+        launch_tanh_rule<T>(ws.Z.as<T>(), outp, ldo, last ? 1 : 0, sh, s);
+      } else {
+        gemm_lap(m->prec, X[l], m->Kx[l], W.Wmain[l].as<float>(), W.Wmain_hi[l].as<uint16_t>(),
+                 W.Wmain_lo[l].as<uint16_t>(), m->Kx[l], ws.V.as<float>(), m->W1, outp, ldo, last ? 1 : 0, sh, s);
+      }
     }
   }
   // orbital GEMMs on the spin-major rows
   const long up_rows = B * m->u * Cp, dn_rows = B * m->dn * Cp;
-  GemmEpi e0;
-  gemm(pr, m->H.f(), m->W1, m->Wo[0].f(), m->Wo_hi[0].h(), m->Wo_lo[0].h(), m->W1, m->O.f(), m->NO, (int)up_rows,
-       m->NO, m->W1, e0, s);
-  gemm(pr, m->H.f() + up_rows * m->W1, m->W1, m->Wo[1].f(), m->Wo_hi[1].h(), m->Wo_lo[1].h(), m->W1,
-       m->O.f() + up_rows * m->NO, m->NO, (int)dn_rows, m->NO, m->W1, e0, s);
-  launch_envelope(m->O.f(), m->NO, r, R, m->kappa.f(), m->wenv.f(), m->A.f(), sh, s);
+  GemmEpiT<T> e0;
+  gemm_t(m, ws.H.as<T>(), m->W1, W.Wo[0].as<T>(), W.Wo_hi[0], W.Wo_lo[0], m->W1, ws.O.as<T>(), m->NO, up_rows, m->NO,
+         m->W1, e0, s);
+  gemm_t(m, ws.H.as<T>() + up_rows * m->W1, m->W1, W.Wo[1].as<T>(), W.Wo_hi[1], W.Wo_lo[1], m->W1,
+         ws.O.as<T>() + up_rows * m->NO, m->NO, dn_rows, m->NO, m->W1, e0, s);
+  launch_envelope<T>(ws.O.as<T>(), m->NO, r, R, W.kappa.as<T>(), W.wenv.as<T>(), ws.A.as<T>(), sh, s);
   {
     Timed t(m, C > 1 ? DW_TIMER_DET_LAP : -1, s);
-    launch_det(m->A.f(), m->ldd.f(), m->sdd.f(), m->gd.f(), m->lapd.f(), m->dfl.u(), m->d.fp64_cond_threshold, sh, s);
+    launch_det<T>(ws.A.as<T>(), m->ldd.as<double>(), m->sdd.as<double>(), m->gd.as<double>(), m->lapd.as<double>(),
+                  m->dcond.as<float>(), sh, s);
   }
-  launch_combine(m->ldd.f(), m->sdd.f(), m->gd.f(), m->lapd.f(), m->dfl.u(), r, R, Z, out_lp, out_phase, out_eloc,
-                 out_flags, sh, s);
+  launch_combine(m->ldd.as<double>(), m->sdd.as<double>(), m->gd.as<double>(), m->lapd.as<double>(),
+                 m->dcond.as<float>(), cond_thr, r, R, Z, out_lp, out_phase, out_eloc, out_flags,
+                 m->kappa_w.as<float>(), sh, s);
   HIP_TRY(hipGetLastError());
   return DW_OK;
+}
+
+dw_status pass(dw_model* m, const float* r, long B, const float* R, const float* Z, int C, hipStream_t s,
+               float* out_lp, float* out_phase, float* out_eloc, uint32_t* out_flags, float cond_thr) {
+  if (m->prec == GEMM_F64) return run_pass<double>(m, r, B, R, Z, C, s, out_lp, out_phase, out_eloc, out_flags, 0.f);
+  return run_pass<float>(m, r, B, R, Z, C, s, out_lp, out_phase, out_eloc, out_flags, cond_thr);
 }
 
 }  // namespace
@@ -398,7 +486,7 @@ dw_status run_pass(dw_model* m, const float* r, long B, const float* R, const fl
 extern "C" {
 
 const char* dw_last_error(void) { return g_err.c_str(); }
-const char* dw_version(void) { return "dwmi 0.1 (gfx950)"; }
+const char* dw_version(void) { return "dwmi 0.2 (gfx950)"; }
 
 dw_status dw_model_set_timing(dw_model* m, int32_t enable) {
   if (!m) return fail(DW_ERR_INVALID, "null model");
@@ -431,14 +519,14 @@ dw_status dw_model_timer_query(dw_model* m, int32_t kind, double* total_ms, int3
 dw_status dw_model_create(const dw_model_desc* desc, dw_model** out) {
   if (!desc || !out) return fail(DW_ERR_INVALID, "null argument");
   const dw_model_desc& d = *desc;
-  if (d.n_el < 2 || d.n_el > 64) return fail(DW_ERR_INVALID, "n_el must be in [2, 64]");
+  if (d.n_el < 2 || d.n_el > 63) return fail(DW_ERR_INVALID, "n_el must be in [2, 63]");
   if (d.n_up < 1 || d.n_up >= d.n_el) return fail(DW_ERR_INVALID, "need 1 <= n_up < n_el (both spin blocks non-empty)");
   if (d.n_ion < 1 || d.n_ion > 64) return fail(DW_ERR_INVALID, "n_ion must be in [1, 64]");
   if (d.n_det < 1) return fail(DW_ERR_INVALID, "n_det must be >= 1");
   if (d.width2 != 32) return fail(DW_ERR_INVALID, "the fused 2-el kernel requires width2 == 32");
-  if (d.width1 < 32 || d.width1 % 128 != 0) return fail(DW_ERR_INVALID, "width1 must be a multiple of 128");
+  if (d.width1 < 128 || d.width1 % 128 != 0) return fail(DW_ERR_INVALID, "width1 must be a multiple of 128");
   if (d.n_iter < 2 || d.n_iter > 4) return fail(DW_ERR_INVALID, "n_iter must be in [2, 4]");
-  if (d.precision < DW_FP32 || d.precision > DW_BF16) return fail(DW_ERR_INVALID, "unknown precision");
+  if (d.precision < DW_FP32 || d.precision > DW_FP64) return fail(DW_ERR_INVALID, "unknown precision");
   dw_model* m = new dw_model();
   m->d = d;
   m->n = d.n_el;
@@ -460,14 +548,6 @@ dw_status dw_model_create(const dw_model_desc* desc, dw_model** out) {
     m->Kx.push_back(pad_to(Kh + 2 * Ke, 32));
     m->Km.push_back(pad_to(2 * Kh, 32));
   }
-  m->Wmain.resize(m->L);
-  m->Wmu.resize(m->L);
-  m->bias.resize(m->L);
-  m->Wmain_hi.resize(m->L);
-  m->Wmain_lo.resize(m->L);
-  m->Wmu_hi.resize(m->L);
-  m->Wmu_lo.resize(m->L);
-  m->X.resize(m->L);
   for (int l = 0; l < m->L; ++l) {
     char buf[64];
     std::snprintf(buf, sizeof buf, "h_el_%d/linear_0", l);
@@ -539,33 +619,74 @@ size_t dw_workspace_size(const dw_model* m, int32_t n_walkers, int32_t mode) {
   const int C = mode ? 3 * m->n + 2 : 1;
   long B = n_walkers;
   if (mode && m->chunk > 0 && B > m->chunk) B = m->chunk;
-  return ws_bytes(m, B, C);
+  return ws_bytes(m, B, C, m->prec == GEMM_F64 ? 8 : 4);
+}
+
+dw_status dw_model_conditioning(dw_model* m, float* out, int32_t n_walkers, dw_stream stream) {
+  if (!m || !out || n_walkers <= 0) return fail(DW_ERR_INVALID, "bad argument");
+  if ((size_t)n_walkers * 4 > m->kappa_w.bytes) return fail(DW_ERR_STATE, "no pass of that size has run");
+  HIP_TRY(hipMemcpyAsync(out, m->kappa_w.p, (size_t)n_walkers * 4, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return DW_OK;
 }
 
 dw_status dw_logpsi(dw_model* m, const float* r, int32_t n_walkers, const float* R, const float* Z,
                     float* out_log_psi_sqr, float* out_phase, dw_stream stream) {
   if (!m || !r || !R || !Z || !out_log_psi_sqr) return fail(DW_ERR_INVALID, "null argument");
   if (n_walkers <= 0) return fail(DW_ERR_INVALID, "n_walkers must be positive");
-  dw_status st = prepare(m);
+  dw_status st = prepare(m, false);
   if (st) return st;
-  return run_pass(m, r, n_walkers, R, Z, 1, (hipStream_t)stream, out_log_psi_sqr, out_phase, nullptr, nullptr);
+  return pass(m, r, n_walkers, R, Z, 1, (hipStream_t)stream, out_log_psi_sqr, out_phase, nullptr, nullptr, 0.f);
 }
 
 dw_status dw_local_energy(dw_model* m, const float* r, int32_t n_walkers, const float* R, const float* Z,
                           float* out_eloc, float* out_log_psi_sqr, uint32_t* out_flags, dw_stream stream) {
   if (!m || !r || !R || !Z || !out_eloc) return fail(DW_ERR_INVALID, "null argument");
   if (n_walkers <= 0) return fail(DW_ERR_INVALID, "n_walkers must be positive");
-  dw_status st = prepare(m);
+  const bool recompute = m->prec != GEMM_F64 && m->d.fp64_cond_threshold > 0.f;
+  dw_status st = prepare(m, recompute);
   if (st) return st;
+  hipStream_t s = (hipStream_t)stream;
   const int C = 3 * m->n + 2;
   const long chunk = m->chunk > 0 ? m->chunk : n_walkers;
+  // flags drive the recompute decision even when the caller does not ask for them
+  uint32_t* flags = out_flags;
+  if (recompute && !flags) {
+    HIP_TRY(m->own_flags.ensure((size_t)n_walkers * 4));
+    flags = m->own_flags.as<uint32_t>();
+  }
   for (long b0 = 0; b0 < n_walkers; b0 += chunk) {
     const long nb = std::min<long>(chunk, n_walkers - b0);
-    st = run_pass(m, r + b0 * m->n * 3, nb, R, Z, C, (hipStream_t)stream,
-                  out_log_psi_sqr ? out_log_psi_sqr + b0 : nullptr, nullptr, out_eloc + b0,
-                  out_flags ? out_flags + b0 : nullptr);
+    st = pass(m, r + b0 * m->n * 3, nb, R, Z, C, s, out_log_psi_sqr ? out_log_psi_sqr + b0 : nullptr, nullptr,
+              out_eloc + b0, flags ? flags + b0 : nullptr, m->d.fp64_cond_threshold);
     if (st) return st;
   }
+  if (!recompute) return DW_OK;
+  // FP64 recompute of the walkers whose conditioning exceeds the threshold (one host synchronisation)
+  std::vector<uint32_t> hf(n_walkers);
+  HIP_TRY(hipMemcpyAsync(hf.data(), flags, (size_t)n_walkers * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  std::vector<int32_t> idx;
+  for (int b = 0; b < n_walkers; ++b)
+    if (hf[b] & DW_FLAG_FP64_DET) idx.push_back(b);
+  if (idx.empty()) return DW_OK;
+  const int cntw = (int)idx.size();
+  HIP_TRY(m->sub_idx.ensure((size_t)cntw * 4));
+  HIP_TRY(m->sub_r.ensure((size_t)cntw * m->n * 3 * 4));
+  HIP_TRY(m->sub_e.ensure((size_t)cntw * 4));
+  HIP_TRY(m->sub_lp.ensure((size_t)cntw * 4));
+  HIP_TRY(hipMemcpyAsync(m->sub_idx.p, idx.data(), (size_t)cntw * 4, hipMemcpyHostToDevice, s));
+  launch_gather_walkers(r, m->sub_idx.as<int32_t>(), cntw, m->n, m->sub_r.as<float>(), s);
+  const long c2 = m->chunk > 0 ? m->chunk : cntw;
+  for (long b0 = 0; b0 < cntw; b0 += c2) {
+    const long nb = std::min<long>(c2, cntw - b0);
+    st = run_pass<double>(m, m->sub_r.as<float>() + b0 * m->n * 3, nb, R, Z, C, s, m->sub_lp.as<float>() + b0,
+                          nullptr, m->sub_e.as<float>() + b0, nullptr, 0.f);
+    if (st) return st;
+  }
+  launch_scatter_values(m->sub_e.as<float>(), m->sub_idx.as<int32_t>(), cntw, out_eloc, s);
+  if (out_log_psi_sqr) launch_scatter_values(m->sub_lp.as<float>(), m->sub_idx.as<int32_t>(), cntw, out_log_psi_sqr, s);
+  // the host index list is copied before this call returns
+  HIP_TRY(hipStreamSynchronize(s));
   return DW_OK;
 }
 
@@ -577,7 +698,7 @@ dw_status dw_mcmc_steps(dw_model* m, float* r, int32_t n_walkers, const float* R
   if (!m || !r || !R || !Z || !log_psi_sqr || !walker_age || !chain) return fail(DW_ERR_INVALID, "null argument");
   if (n_walkers <= 0 || n_steps < 0) return fail(DW_ERR_INVALID, "bad sizes");
   if (n_steps > 0 && (!normals || !uniforms)) return fail(DW_ERR_INVALID, "draws required");
-  dw_status st = prepare(m);
+  dw_status st = prepare(m, false);
   if (st) return st;
   hipStream_t s = (hipStream_t)stream;
   const long B = n_walkers;
@@ -586,18 +707,18 @@ dw_status dw_mcmc_steps(dw_model* m, float* r, int32_t n_walkers, const float* R
   HIP_TRY(m->lpprop.ensure((size_t)B * 4));
   HIP_TRY(m->cnt.ensure((size_t)std::max(n_steps, 1) * 4));
   // log_psi_sqr is re-evaluated on entry (mcmc.py:396)
-  st = run_pass(m, r, B, R, Z, 1, s, log_psi_sqr, nullptr, nullptr, nullptr);
+  st = pass(m, r, B, R, Z, 1, s, log_psi_sqr, nullptr, nullptr, nullptr, 0.f);
   if (st) return st;
   if (n_steps == 0) return DW_OK;
-  launch_fill_i32(m->cnt.i(), 0, n_steps, s);
+  launch_fill_i32(m->cnt.as<int32_t>(), 0, n_steps, s);
   for (int k = 0; k < n_steps; ++k) {
-    launch_propose(r, normals + (size_t)k * cnt3, chain, m->rprop.f(), cnt3, s);
-    st = run_pass(m, m->rprop.f(), B, R, Z, 1, s, m->lpprop.f(), nullptr, nullptr, nullptr);
+    launch_propose(r, normals + (size_t)k * cnt3, chain, m->rprop.as<float>(), cnt3, s);
+    st = pass(m, m->rprop.as<float>(), B, R, Z, 1, s, m->lpprop.as<float>(), nullptr, nullptr, nullptr, 0.f);
     if (st) return st;
-    launch_accept(r, m->rprop.f(), log_psi_sqr, m->lpprop.f(), uniforms + (size_t)k * B, walker_age, max_age,
-                  m->cnt.i() + k, out_flags, (int)B, m->n, s);
-    launch_chain_update(chain, m->cnt.i() + k, (int)B, out_acc ? out_acc + k : nullptr, stepsize_update_interval,
-                        target_acceptance, min_stepsize, max_stepsize, s);
+    launch_accept(r, m->rprop.as<float>(), log_psi_sqr, m->lpprop.as<float>(), uniforms + (size_t)k * B, walker_age,
+                  max_age, m->cnt.as<int32_t>() + k, out_flags, (int)B, m->n, s);
+    launch_chain_update(chain, m->cnt.as<int32_t>() + k, (int)B, out_acc ? out_acc + k : nullptr,
+                        stepsize_update_interval, target_acceptance, min_stepsize, max_stepsize, s);
   }
   HIP_TRY(hipGetLastError());
   return DW_OK;

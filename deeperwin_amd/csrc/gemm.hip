@@ -17,7 +17,8 @@ namespace dwmi {
 
 enum { EPI_BIAS = 1, EPI_V = 2, EPI_TANH = 4, EPI_PERM = 8 };
 
-__device__ __forceinline__ long out_row(long row, const GemmEpi& ep) {
+template <typename T>
+__device__ __forceinline__ long out_row(long row, const GemmEpiT<T>& ep) {
   // row = (b * n + i) * C + ch  ->  spin-major order
   const long ch = row % ep.C;
   const long bi = row / ep.C;
@@ -27,8 +28,11 @@ __device__ __forceinline__ long out_row(long row, const GemmEpi& ep) {
   return (long)ep.B * ep.u * ep.C + (b * dn + (i - ep.u)) * ep.C + ch;
 }
 
-template <int FLAGS>
-__device__ __forceinline__ void epi_store(float v, long r, int c, float* __restrict__ Cm, int ldc, const GemmEpi& ep) {
+__device__ __forceinline__ float act_tanh(float v) { return tanhf(v); }
+__device__ __forceinline__ double act_tanh(double v) { return tanh(v); }
+
+template <int FLAGS, typename T>
+__device__ __forceinline__ void epi_store(T v, long r, int c, T* __restrict__ Cm, int ldc, const GemmEpiT<T>& ep) {
   if ((FLAGS & EPI_BIAS) && ep.bias) {
     if ((r % ep.C) == 0) v += ep.bias[c];
   }
@@ -36,7 +40,7 @@ __device__ __forceinline__ void epi_store(float v, long r, int c, float* __restr
     const long vr = (r / ep.nC) * ep.C + (r % ep.C);
     v += ep.V[vr * ep.ldv + c];
   }
-  if ((FLAGS & EPI_TANH) && ep.act_tanh) v = tanhf(v);
+  if ((FLAGS & EPI_TANH) && ep.act_tanh) v = act_tanh(v);
   const long orow = ((FLAGS & EPI_PERM) && ep.permute) ? out_row(r, ep) : r;
   Cm[orow * ldc + c] = v;
 }
@@ -126,7 +130,7 @@ __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ A, i
       for (int j = 0; j < 16; ++j) {
         const long r = row0 + wm * 64 + mi * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
         const int c = col0 + wn * 64 + ni * 32 + (lane & 31);
-        if (r < M && c < N) epi_store<FLAGS>(acc[mi][ni][j], r, c, Cm, ldc, ep);
+        if (r < M && c < N) epi_store<FLAGS, float>(acc[mi][ni][j], r, c, Cm, ldc, ep);
       }
 }
 
@@ -243,8 +247,79 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const float* __restrict__ A, 
       for (int j = 0; j < 16; ++j) {
         const long r = row0 + wm * 64 + mi * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
         const int c = col0 + wn * 64 + ni * 32 + (lane & 31);
-        if (r < M && c < N) epi_store<FLAGS>(acc[mi][ni][j], r, c, Cm, ldc, ep);
+        if (r < M && c < N) epi_store<FLAGS, float>(acc[mi][ni][j], r, c, Cm, ldc, ep);
       }
+}
+
+// ------------------------------------------------------------------------------------------------
+// FP64 (VALU FMA): 64 x 64 tile per 256-thread workgroup, thread (ty, tx) owns rows ty + 16 i and
+// columns tx + 16 j (i, j < 4), BK = 16.
+// ------------------------------------------------------------------------------------------------
+template <int FLAGS>
+__global__ __launch_bounds__(256) void k_gemm_f64(const double* __restrict__ A, int lda, const double* __restrict__ Bt,
+                                                  int ldb, double* __restrict__ Cm, int ldc, int M, int N, int K,
+                                                  int tilesN, int tilesTotal, GemmEpiD ep) {
+  constexpr int BM = 64, BN = 64, BK = 16;
+  __shared__ double As[BK][BM + 1];
+  __shared__ double Bs[BK][BN + 1];
+  const int tile = tile_index(blockIdx.x, tilesTotal);
+  if (tile >= tilesTotal) return;
+  const long row0 = (long)(tile / tilesN) * BM;
+  const int col0 = (tile % tilesN) * BN;
+  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+  double acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+      const int idx = tid + 256 * it, m = idx >> 3, kq = (idx & 7) * 2;
+      long gr = row0 + m;
+      if (gr > M - 1) gr = M - 1;
+      const double2 a2 = *(const double2*)(A + gr * lda + k0 + kq);
+      As[kq][m] = a2.x;
+      As[kq + 1][m] = a2.y;
+      long gc = col0 + m;
+      const double2 b2 = *(const double2*)(Bt + gc * ldb + k0 + kq);
+      Bs[kq][m] = b2.x;
+      Bs[kq + 1][m] = b2.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; ++kk) {
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[kk][ty + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const long r = row0 + ty + 16 * i;
+      const int c = col0 + tx + 16 * j;
+      if (r < M && c < N) epi_store<FLAGS, double>(acc[i][j], r, c, Cm, ldc, ep);
+    }
+}
+
+void gemm_f64(const double* A, int lda, const double* Bt, int ldb, double* C, int ldc, int M, int N, int K,
+              const GemmEpiD& ep, hipStream_t s) {
+  if (M <= 0 || N <= 0) return;
+  const int tilesM = (M + 63) / 64, tilesN = (N + 63) / 64;
+  const int total = tilesM * tilesN;
+  const int grid = (total + 7) / 8 * 8;
+  // one generic instance: flags are tested at run time inside epi_store
+  hipLaunchKernelGGL((k_gemm_f64<EPI_BIAS | EPI_V | EPI_TANH | EPI_PERM>), dim3(grid), dim3(256), 0, s, A, lda, Bt,
+                     ldb, C, ldc, M, N, K, tilesN, total, ep);
 }
 
 template <int FLAGS>

@@ -15,13 +15,22 @@ namespace dwmi {
 
 static constexpr int MAXL = 4;  // FermiNet iterations supported by the fused 2-el kernel
 
+// Scalar-type generic math (T = float for the FP32/BF16 forms, double for the FP64 form).
+__device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float dtanh(float x) { return tanhf(x); }
+__device__ __forceinline__ double dtanh(double x) { return tanh(x); }
+__device__ __forceinline__ float dexp(float x) { return expf(x); }
+__device__ __forceinline__ double dexp(double x) { return exp(x); }
+
 // ------------------------------------------------------------------------------------------------
 // Layer-0 one-electron features: per ion [|r_i - R_I|, r_i - R_I] (utils.py:262-273,
 // input_features.py:41-68, 237-239), with their tangents (only along electron i) and Laplacians
 // (Laplacian of |x| in 3D is 2/|x|; of a coordinate difference 0).
 // ------------------------------------------------------------------------------------------------
-__global__ void k_features0(const float* __restrict__ r, const float* __restrict__ R, float* __restrict__ X0,
-                            int Kx0, Shape sh) {
+template <typename T>
+__global__ void k_features0(const float* __restrict__ r, const float* __restrict__ R, T* __restrict__ X0, int Kx0,
+                            Shape sh) {
   const long total = (long)sh.B * sh.n * sh.Cp * Kx0;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
@@ -32,32 +41,35 @@ __global__ void k_features0(const float* __restrict__ r, const float* __restrict
   const int i = (int)(bi % sh.n);
   const int I4 = 4 * sh.I;
   if (ch >= sh.C) {  // padding channel rows stay zero
-    X0[t] = 0.f;
+    X0[t] = T(0);
     return;
   }
   if (col >= I4 && col < I4 + 8) return;  // pair-mean columns: written by k_twoel
-  float v = 0.f;
+  T v = T(0);
   if (col < I4) {
     const int ion = col >> 2, f = col & 3;
     const float* ri = r + bi * 3;
-    const float d[3] = {ri[0] - R[ion * 3 + 0], ri[1] - R[ion * 3 + 1], ri[2] - R[ion * 3 + 2]};
-    const float rho = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const T d[3] = {(T)ri[0] - (T)R[ion * 3 + 0], (T)ri[1] - (T)R[ion * 3 + 1], (T)ri[2] - (T)R[ion * 3 + 2]};
+    const T rho = dsqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     if (ch == 0) {
       v = (f == 0) ? rho : d[f - 1];
     } else if (ch == sh.C - 1) {
-      v = (f == 0) ? 2.f / rho : 0.f;
+      v = (f == 0) ? T(2) / rho : T(0);
     } else {
       const int k = ch - 1, e = k / 3, c = k % 3;
-      if (e == i) v = (f == 0) ? d[c] / rho : ((f - 1 == c) ? 1.f : 0.f);
+      if (e == i) v = (f == 0) ? d[c] / rho : ((f - 1 == c) ? T(1) : T(0));
     }
   }
   X0[t] = v;
 }
 
-void launch_features0(const float* r, const float* R, float* X0, int Kx0, const Shape& sh, hipStream_t s) {
+template <typename T>
+void launch_features0(const float* r, const float* R, T* X0, int Kx0, const Shape& sh, hipStream_t s) {
   const long total = (long)sh.B * sh.n * sh.Cp * Kx0;
-  hipLaunchKernelGGL(k_features0, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, r, R, X0, Kx0, sh);
+  hipLaunchKernelGGL(k_features0<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, r, R, X0, Kx0, sh);
 }
+template void launch_features0<float>(const float*, const float*, float*, int, const Shape&, hipStream_t);
+template void launch_features0<double>(const float*, const float*, double*, int, const Shape&, hipStream_t);
 
 // ------------------------------------------------------------------------------------------------
 // Fused two-electron stream.  One workgroup per (walker b, group of 8 electrons i); 256 threads =
@@ -71,15 +83,16 @@ void launch_features0(const float* r, const float* R, float* X0, int Kx0, const 
 //   value / Laplacian rows : mean over j in the spin block
 //   tangent row (e, c)     : T_c[i, e] / n_s(e) in block s(e)  (e != i);  -sum_j T_c[i, j] / n_s (e == i)
 // ------------------------------------------------------------------------------------------------
+template <typename T>
 struct TwoElOut {
-  float* X[MAXL];
+  T* X[MAXL];
   int Kx[MAXL];
   int Kh[MAXL];
 };
 
-template <bool LAP, int L>
-__global__ __launch_bounds__(256) void k_twoel(const float* __restrict__ r, TwoElOut out,
-                                               const float* __restrict__ W2e, const float* __restrict__ b2e,
+template <bool LAP, int L, typename T>
+__global__ __launch_bounds__(256) void k_twoel(const float* __restrict__ r, TwoElOut<T> out,
+                                               const T* __restrict__ W2e, const T* __restrict__ b2e,
                                                Shape sh) {
   constexpr int NCH = LAP ? 5 : 1;
   constexpr int NL2 = (L > 1) ? L - 1 : 1;
@@ -90,51 +103,51 @@ __global__ __launch_bounds__(256) void k_twoel(const float* __restrict__ r, TwoE
   const int i = (int)(blockIdx.x % groups) * 8 + il;
   const bool active = i < n;
   const long bi = b * n + (active ? i : 0);
-  __shared__ float xs[8][NCH][32];
+  __shared__ T xs[8][NCH][32];
 
-  float w[NL2][32], bb[NL2];
+  T w[NL2][32], bb[NL2];
 #pragma unroll
   for (int l2 = 0; l2 < L - 1; ++l2) {
     bb[l2] = b2e[l2 * 32 + f];
 #pragma unroll
     for (int k = 0; k < 32; ++k) w[l2][k] = W2e[(l2 * 32 + k) * 32 + f];
   }
-  float acc0[L][NCH], acc1[L][NCH];
+  T acc0[L][NCH], acc1[L][NCH];
 #pragma unroll
   for (int l = 0; l < L; ++l)
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) acc0[l][c] = acc1[l][c] = 0.f;
+    for (int c = 0; c < NCH; ++c) acc0[l][c] = acc1[l][c] = T(0);
 
   const float* rb = r + b * n * 3;
-  const float rix = active ? rb[i * 3 + 0] : 0.f, riy = active ? rb[i * 3 + 1] : 0.f,
-              riz = active ? rb[i * 3 + 2] : 0.f;
-  const float inv_up = 1.f / (float)u, inv_dn = 1.f / (float)(n - u);
-  const float rs2 = 0.70710678118654752440f;
+  const T rix = active ? (T)rb[i * 3 + 0] : T(0), riy = active ? (T)rb[i * 3 + 1] : T(0),
+          riz = active ? (T)rb[i * 3 + 2] : T(0);
+  const T inv_up = T(1) / (T)u, inv_dn = T(1) / (T)(n - u);
+  const T rs2 = (T)0.70710678118654752440;
 
   for (int j = 0; j < n; ++j) {
     const int sj = (j < u) ? 0 : 1;
-    const float inv_s = sj == 0 ? inv_up : inv_dn;
+    const T inv_s = sj == 0 ? inv_up : inv_dn;
     const bool offdiag = active && j != i;
-    float x[NCH];
+    T x[NCH];
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) x[c] = 0.f;
+    for (int c = 0; c < NCH; ++c) x[c] = T(0);
     if (offdiag && f < 4) {
-      const float dx = rb[j * 3 + 0] - rix, dy = rb[j * 3 + 1] - riy, dz = rb[j * 3 + 2] - riz;
-      const float rho = sqrtf(dx * dx + dy * dy + dz * dz);
+      const T dx = (T)rb[j * 3 + 0] - rix, dy = (T)rb[j * 3 + 1] - riy, dz = (T)rb[j * 3 + 2] - riz;
+      const T rho = dsqrt(dx * dx + dy * dy + dz * dz);
       if (f == 0) {
         x[0] = rho;
         if constexpr (LAP) {
           x[1] = dx / rho;
           x[2] = dy / rho;
           x[3] = dz / rho;
-          x[4] = 4.f / rho;
+          x[4] = T(4) / rho;
         }
       } else {
         x[0] = (f == 1) ? dx : (f == 2 ? dy : dz);
         if constexpr (LAP) {
-          x[1] = (f == 1) ? 1.f : 0.f;
-          x[2] = (f == 2) ? 1.f : 0.f;
-          x[3] = (f == 3) ? 1.f : 0.f;
+          x[1] = (f == 1) ? T(1) : T(0);
+          x[2] = (f == 2) ? T(1) : T(0);
+          x[3] = (f == 3) ? T(1) : T(0);
         }
       }
     }
@@ -149,9 +162,9 @@ __global__ __launch_bounds__(256) void k_twoel(const float* __restrict__ r, TwoE
       if (offdiag && f < 4) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          float* row = out.X[0] + (bi * Cp + 1 + 3 * j + c) * (long)out.Kx[0] + out.Kh[0];
+          T* row = out.X[0] + (bi * Cp + 1 + 3 * j + c) * (long)out.Kx[0] + out.Kh[0];
           row[sj * 4 + f] = x[1 + c] * inv_s;
-          row[(1 - sj) * 4 + f] = 0.f;
+          row[(1 - sj) * 4 + f] = T(0);
         }
       }
     }
@@ -162,29 +175,28 @@ __global__ __launch_bounds__(256) void k_twoel(const float* __restrict__ r, TwoE
 #pragma unroll
       for (int c = 0; c < NCH; ++c) xs[il][c][f] = x[c];
       __syncthreads();
-      float z[NCH];
+      T z[NCH];
       z[0] = bb[l2];
 #pragma unroll
-      for (int c = 1; c < NCH; ++c) z[c] = 0.f;
+      for (int c = 1; c < NCH; ++c) z[c] = T(0);
 #pragma unroll
       for (int k4 = 0; k4 < 32; k4 += 4) {
         if (k4 < kin) {
 #pragma unroll
-          for (int c = 0; c < NCH; ++c) {
-            const float4 xv = *(const float4*)&xs[il][c][k4];
-            z[c] += xv.x * w[l2][k4] + xv.y * w[l2][k4 + 1] + xv.z * w[l2][k4 + 2] + xv.w * w[l2][k4 + 3];
-          }
+          for (int c = 0; c < NCH; ++c)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) z[c] += xs[il][c][k4 + q] * w[l2][k4 + q];
         }
       }
-      const float th = tanhf(z[0]);
-      float y[NCH];
+      const T th = dtanh(z[0]);
+      T y[NCH];
       y[0] = th;
       if constexpr (LAP) {
-        const float d1 = 1.f - th * th, d2 = -2.f * th * d1;
+        const T d1 = T(1) - th * th, d2 = T(-2) * th * d1;
         y[1] = d1 * z[1];
         y[2] = d1 * z[2];
         y[3] = d1 * z[3];
-        y[4] = d1 * z[4] + d2 * 2.f * (z[1] * z[1] + z[2] * z[2] + z[3] * z[3]);
+        y[4] = d1 * z[4] + d2 * T(2) * (z[1] * z[1] + z[2] * z[2] + z[3] * z[3]);
       }
       if (kin == 32) {
 #pragma unroll
@@ -204,9 +216,9 @@ __global__ __launch_bounds__(256) void k_twoel(const float* __restrict__ r, TwoE
           const int l = l2 + 1;
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            float* row = out.X[l] + (bi * Cp + 1 + 3 * j + c) * (long)out.Kx[l] + out.Kh[l];
+            T* row = out.X[l] + (bi * Cp + 1 + 3 * j + c) * (long)out.Kx[l] + out.Kh[l];
             row[sj * 32 + f] = y[1 + c] * inv_s;
-            row[(1 - sj) * 32 + f] = 0.f;
+            row[(1 - sj) * 32 + f] = T(0);
           }
         }
       }
@@ -217,13 +229,13 @@ __global__ __launch_bounds__(256) void k_twoel(const float* __restrict__ r, TwoE
   for (int l = 0; l < L; ++l) {
     const int Ke = (l == 0) ? 4 : 32;
     if (f >= Ke) continue;
-    float* X = out.X[l];
+    T* X = out.X[l];
     const long Kx = out.Kx[l];
     const int Kh = out.Kh[l];
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-      const float inv = (s == 0) ? inv_up : inv_dn;
-      float a[NCH];
+      const T inv = (s == 0) ? inv_up : inv_dn;
+      T a[NCH];
 #pragma unroll
       for (int c = 0; c < NCH; ++c) a[c] = (s == 0) ? acc0[l][c] : acc1[l][c];
       X[(bi * Cp) * Kx + Kh + s * Ke + f] = a[0] * inv;
@@ -236,35 +248,41 @@ __global__ __launch_bounds__(256) void k_twoel(const float* __restrict__ r, TwoE
   }
 }
 
-template <bool LAP>
-static void twoel_dispatch(const float* r, const TwoElOut& o, const float* W2e, const float* b2e, const Shape& sh,
+template <bool LAP, typename T>
+static void twoel_dispatch(const float* r, const TwoElOut<T>& o, const T* W2e, const T* b2e, const Shape& sh,
                            hipStream_t s) {
   const unsigned grid = (unsigned)(sh.B * ((sh.n + 7) / 8));
   switch (sh.L) {
-    case 2: hipLaunchKernelGGL((k_twoel<LAP, 2>), dim3(grid), dim3(256), 0, s, r, o, W2e, b2e, sh); break;
-    case 3: hipLaunchKernelGGL((k_twoel<LAP, 3>), dim3(grid), dim3(256), 0, s, r, o, W2e, b2e, sh); break;
-    default: hipLaunchKernelGGL((k_twoel<LAP, 4>), dim3(grid), dim3(256), 0, s, r, o, W2e, b2e, sh); break;
+    case 2: hipLaunchKernelGGL((k_twoel<LAP, 2, T>), dim3(grid), dim3(256), 0, s, r, o, W2e, b2e, sh); break;
+    case 3: hipLaunchKernelGGL((k_twoel<LAP, 3, T>), dim3(grid), dim3(256), 0, s, r, o, W2e, b2e, sh); break;
+    default: hipLaunchKernelGGL((k_twoel<LAP, 4, T>), dim3(grid), dim3(256), 0, s, r, o, W2e, b2e, sh); break;
   }
 }
 
-void launch_twoel(const float* r, float* const* X, const int* Kx, const int* Kh, const float* W2e, const float* b2e,
+template <typename T>
+void launch_twoel(const float* r, T* const* X, const int* Kx, const int* Kh, const T* W2e, const T* b2e,
                   const Shape& sh, hipStream_t s) {
-  TwoElOut o;
+  TwoElOut<T> o;
   for (int l = 0; l < MAXL; ++l) {
     o.X[l] = l < sh.L ? X[l] : nullptr;
     o.Kx[l] = l < sh.L ? Kx[l] : 0;
     o.Kh[l] = l < sh.L ? Kh[l] : 0;
   }
   if (sh.C > 1)
-    twoel_dispatch<true>(r, o, W2e, b2e, sh, s);
+    twoel_dispatch<true, T>(r, o, W2e, b2e, sh, s);
   else
-    twoel_dispatch<false>(r, o, W2e, b2e, sh, s);
+    twoel_dispatch<false, T>(r, o, W2e, b2e, sh, s);
 }
+template void launch_twoel<float>(const float*, float* const*, const int*, const int*, const float*, const float*,
+                                  const Shape&, hipStream_t);
+template void launch_twoel<double>(const float*, double* const*, const int*, const int*, const double*,
+                                   const double*, const Shape&, hipStream_t);
 
 // ------------------------------------------------------------------------------------------------
 // Spin means of h over electrons (ferminet_embedding.py:61-62, 72): MU[b, ch] = [mean_{i<u} h, mean_{i>=u} h].
 // ------------------------------------------------------------------------------------------------
-__global__ void k_spin_mean(const float* __restrict__ X, int Kx, int Kh, float* __restrict__ MU, int Km, Shape sh) {
+template <typename T>
+__global__ void k_spin_mean(const T* __restrict__ X, int Kx, int Kh, T* __restrict__ MU, int Km, Shape sh) {
   const long total = (long)sh.B * sh.Cp * Km;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
@@ -272,22 +290,67 @@ __global__ void k_spin_mean(const float* __restrict__ X, int Kx, int Kh, float* 
   const long row = t / Km;
   const int ch = (int)(row % sh.Cp);
   const long b = row / sh.Cp;
-  float v = 0.f;
+  T v = T(0);
   if (f < 2 * Kh) {
     const int s = f >= Kh ? 1 : 0;
     const int fi = f - s * Kh;
     const int i0 = s ? sh.u : 0, i1 = s ? sh.n : sh.u;
-    float acc = 0.f;
+    T acc = T(0);
     for (int i = i0; i < i1; ++i) acc += X[((b * sh.n + i) * sh.Cp + ch) * (long)Kx + fi];
-    v = acc / (float)(i1 - i0);
+    v = acc / (T)(i1 - i0);
   }
   MU[t] = v;
 }
 
-void launch_spin_mean(const float* X, int Kx, int Kh, float* MU, int Km, const Shape& sh, hipStream_t s) {
+template <typename T>
+void launch_spin_mean(const T* X, int Kx, int Kh, T* MU, int Km, const Shape& sh, hipStream_t s) {
   const long total = (long)sh.B * sh.Cp * Km;
-  hipLaunchKernelGGL(k_spin_mean, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, X, Kx, Kh, MU, Km, sh);
+  hipLaunchKernelGGL(k_spin_mean<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, X, Kx, Kh, MU, Km, sh);
 }
+template void launch_spin_mean<float>(const float*, int, int, float*, int, const Shape&, hipStream_t);
+template void launch_spin_mean<double>(const double*, int, int, double*, int, const Shape&, hipStream_t);
+
+// ------------------------------------------------------------------------------------------------
+// tanh with forward-mode Laplacian as a separate pass (used by the FP64 form, whose GEMM has no fused
+// epilogue): y = tanh z; dy_k = t' dz_k; Lap y = t' Lap z + t'' sum_k dz_k^2.  Padding channels -> 0.
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+__global__ void k_tanh_rule(const T* __restrict__ Z, T* __restrict__ out, int ldo, int permute, Shape sh) {
+  const long total = (long)sh.B * sh.n * sh.W1;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int f = (int)(t % sh.W1);
+  const long bi = t / sh.W1;
+  const int i = (int)(bi % sh.n);
+  const long b = bi / sh.n;
+  const int C = sh.C, Cp = sh.Cp;
+  const T* z = Z + bi * Cp * (long)sh.W1 + f;
+  long obase;
+  if (permute) {
+    const int dn = sh.n - sh.u;
+    obase = (i < sh.u) ? (b * sh.u + i) * Cp : (long)sh.B * sh.u * Cp + (b * dn + (i - sh.u)) * Cp;
+  } else {
+    obase = bi * Cp;
+  }
+  const T th = dtanh(z[0]);
+  const T d1 = T(1) - th * th, d2 = T(-2) * th * d1;
+  out[obase * ldo + f] = th;
+  T S = T(0);
+  for (int ch = 1; ch < C - 1; ++ch) {
+    const T zc = z[(long)ch * sh.W1];
+    S += zc * zc;
+    out[(obase + ch) * ldo + f] = d1 * zc;
+  }
+  out[(obase + C - 1) * ldo + f] = d1 * z[(long)(C - 1) * sh.W1] + d2 * S;
+  for (int ch = C; ch < Cp; ++ch) out[(obase + ch) * ldo + f] = T(0);
+}
+
+template <typename T>
+void launch_tanh_rule(const T* Z, T* out, int ldo, int permute, const Shape& sh, hipStream_t s) {
+  const long total = (long)sh.B * sh.n * sh.W1;
+  hipLaunchKernelGGL(k_tanh_rule<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, Z, out, ldo, permute, sh);
+}
+template void launch_tanh_rule<double>(const double*, double*, int, int, const Shape&, hipStream_t);
 
 // ------------------------------------------------------------------------------------------------
 // Envelope x backflow (envelope_orbitals.py:91-127): A[b, d, i, k] = env_s[i, c] * bf[i, c], c = d n + k,
@@ -296,9 +359,9 @@ void launch_spin_mean(const float* X, int Kx, int Kh, float* MU, int Km, const S
 //   Lap A = Lap(env) bf + 2 grad_i(env) . grad_i(bf) + env Lap(bf),   Lap e^{-k rho} = (k^2 - 2k/rho) e^{-k rho}.
 // O rows are spin-major (up block then down block, Cp rows per electron); A is [B, D, C, n, n].
 // ------------------------------------------------------------------------------------------------
-__global__ void k_envelope(const float* __restrict__ O, int ldo, const float* __restrict__ r,
-                           const float* __restrict__ R, const float* __restrict__ kappa,
-                           const float* __restrict__ wenv, float* __restrict__ A, Shape sh) {
+template <typename T>
+__global__ void k_envelope(const T* __restrict__ O, int ldo, const float* __restrict__ r, const float* __restrict__ R,
+                           const T* __restrict__ kappa, const T* __restrict__ wenv, T* __restrict__ A, Shape sh) {
   const int n = sh.n, NO = sh.D * n, C = sh.C;
   const long total = (long)sh.B * n * NO;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -312,48 +375,53 @@ __global__ void k_envelope(const float* __restrict__ O, int ldo, const float* __
   const int Cp = sh.Cp;
   const long orow = (s == 0) ? (b * sh.u + i) * Cp : (long)sh.B * sh.u * Cp + (b * dn + (i - sh.u)) * Cp;
   const float* ri = r + bi * 3;
-  const float* kap = kappa + (long)s * sh.I * NO;
-  const float* we = wenv + (long)s * sh.I * NO;
-  float env = 0.f, g0 = 0.f, g1 = 0.f, g2 = 0.f, lenv = 0.f;
+  const T* kap = kappa + (long)s * sh.I * NO;
+  const T* we = wenv + (long)s * sh.I * NO;
+  T env = T(0), g0 = T(0), g1 = T(0), g2 = T(0), lenv = T(0);
   for (int I = 0; I < sh.I; ++I) {
-    const float dx = ri[0] - R[I * 3 + 0], dy = ri[1] - R[I * 3 + 1], dz = ri[2] - R[I * 3 + 2];
-    const float rho = sqrtf(dx * dx + dy * dy + dz * dz);
-    const float k = kap[I * NO + col];
-    const float e = we[I * NO + col] * expf(-k * rho);
+    const T dx = (T)ri[0] - (T)R[I * 3 + 0], dy = (T)ri[1] - (T)R[I * 3 + 1], dz = (T)ri[2] - (T)R[I * 3 + 2];
+    const T rho = dsqrt(dx * dx + dy * dy + dz * dz);
+    const T k = kap[I * NO + col];
+    const T e = we[I * NO + col] * dexp(-k * rho);
     env += e;
     if (C > 1) {
-      const float coef = -k * e / rho;
+      const T coef = -k * e / rho;
       g0 += coef * dx;
       g1 += coef * dy;
       g2 += coef * dz;
-      lenv += e * (k * k - 2.f * k / rho);
+      lenv += e * (k * k - T(2) * k / rho);
     }
   }
   const int d = col / n, k = col % n;
   const long nn = (long)n * n;
-  float* Ab = A + ((b * sh.D + d) * C) * nn + (long)i * n + k;
-  const float bf0 = O[orow * ldo + col];
+  T* Ab = A + ((b * sh.D + d) * C) * nn + (long)i * n + k;
+  const T bf0 = O[orow * ldo + col];
   Ab[0] = env * bf0;
   if (C > 1) {
     for (int ch = 1; ch < C - 1; ++ch) {
       const int e = (ch - 1) / 3, c = (ch - 1) % 3;
-      float v = env * O[(orow + ch) * ldo + col];
+      T v = env * O[(orow + ch) * ldo + col];
       if (e == i) v += (c == 0 ? g0 : (c == 1 ? g1 : g2)) * bf0;
       Ab[(long)ch * nn] = v;
     }
-    const float t0 = O[(orow + 1 + 3 * i) * ldo + col];
-    const float t1 = O[(orow + 2 + 3 * i) * ldo + col];
-    const float t2 = O[(orow + 3 + 3 * i) * ldo + col];
-    Ab[(long)(C - 1) * nn] = lenv * bf0 + env * O[(orow + C - 1) * ldo + col] + 2.f * (g0 * t0 + g1 * t1 + g2 * t2);
+    const T t0 = O[(orow + 1 + 3 * i) * ldo + col];
+    const T t1 = O[(orow + 2 + 3 * i) * ldo + col];
+    const T t2 = O[(orow + 3 + 3 * i) * ldo + col];
+    Ab[(long)(C - 1) * nn] = lenv * bf0 + env * O[(orow + C - 1) * ldo + col] + T(2) * (g0 * t0 + g1 * t1 + g2 * t2);
   }
 }
 
-void launch_envelope(const float* O, int ldo, const float* r, const float* R, const float* kappa, const float* wenv,
-                     float* A, const Shape& sh, hipStream_t s) {
+template <typename T>
+void launch_envelope(const T* O, int ldo, const float* r, const float* R, const T* kappa, const T* wenv, T* A,
+                     const Shape& sh, hipStream_t s) {
   const long total = (long)sh.B * sh.n * sh.D * sh.n;
-  hipLaunchKernelGGL(k_envelope, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, O, ldo, r, R, kappa, wenv,
+  hipLaunchKernelGGL(k_envelope<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, O, ldo, r, R, kappa, wenv,
                      A, sh);
 }
+template void launch_envelope<float>(const float*, int, const float*, const float*, const float*, const float*, float*,
+                                     const Shape&, hipStream_t);
+template void launch_envelope<double>(const double*, int, const float*, const float*, const double*, const double*,
+                                      double*, const Shape&, hipStream_t);
 
 // ------------------------------------------------------------------------------------------------
 // Determinants, in FP64.  One wavefront per (walker, det) matrix; lane j holds column j in registers.
@@ -363,8 +431,8 @@ void launch_envelope(const float* O, int ldo, const float* r, const float* R, co
 // formed column per lane (forward / back substitution against the broadcast factors) and stored in
 // LDS; the tangent channels are then processed several per pass (lane = (channel, column)):
 //   Y_k = M^-1 dM_k,  g_k = tr Y_k,  q_k = tr(Y_k^2),   Lap log|det| = tr(M^-1 Lap M) - sum_k q_k.
-// The 1-norm condition number |M|_1 |M^-1|_1 (Laplacian mode) or max|U_pp| / min|U_pp| (value mode)
-// is compared with `cond_thr` and reported through dflags.
+// The 1-norm condition number |M|_1 |M^-1|_1 (Laplacian mode) or the pivot ratio max|U_pp| / min|U_pp|
+// (value mode) is written per matrix to dcond; results are FP64.
 // ------------------------------------------------------------------------------------------------
 __device__ __forceinline__ double readlane_d(double v, int lane) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
@@ -459,14 +527,14 @@ __device__ __forceinline__ void finish_slogdet(double mydiag, int swaps, bool si
 }
 
 // Value mode: 4 matrices per 256-thread workgroup, no LDS.
-template <int NMAX>
-__global__ __launch_bounds__(256) void k_det_value(const float* __restrict__ A, float* __restrict__ ld,
-                                                   float* __restrict__ sd, uint32_t* __restrict__ dflags,
-                                                   float cond_thr, long nmat, int n, long stride) {
+template <int NMAX, typename TA>
+__global__ __launch_bounds__(256) void k_det_value(const TA* __restrict__ A, double* __restrict__ ld,
+                                                   double* __restrict__ sd, float* __restrict__ dcond, long nmat,
+                                                   int n, long stride) {
   const int lane = threadIdx.x & 63;
   const long bd = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (bd >= nmat) return;
-  const float* Ab = A + bd * stride;
+  const TA* Ab = A + bd * stride;
   const bool act = lane < n;
   double col[NMAX];
   int perm[NMAX];
@@ -480,18 +548,18 @@ __global__ __launch_bounds__(256) void k_det_value(const float* __restrict__ A, 
   const double umax = wave_max_d(act ? fabs(mydiag) : 0.0);
   const double umin = wave_min_d(act ? fabs(mydiag) : 1e300);
   if (lane == 0) {
-    ld[bd] = (float)logabs;
-    sd[bd] = (float)sgn;
-    dflags[bd] = (cond_thr > 0.f && !(umax <= (double)cond_thr * umin)) ? 1u : 0u;
+    ld[bd] = logabs;
+    sd[bd] = sgn;
+    dcond[bd] = (float)(umax / umin);
   }
 }
 
 // Laplacian mode: WPB matrices per workgroup, LDS holds M^-1 and the per-pass Y_k of every wave.
-template <int NMAX, int WPB>
-__global__ __launch_bounds__(64 * WPB) void k_det_lap(const float* __restrict__ A, float* __restrict__ ld,
-                                                      float* __restrict__ sd, float* __restrict__ g,
-                                                      float* __restrict__ lapd, uint32_t* __restrict__ dflags,
-                                                      float cond_thr, long nmat, int n, int C) {
+template <int NMAX, int WPB, typename TA>
+__global__ __launch_bounds__(64 * WPB) void k_det_lap(const TA* __restrict__ A, double* __restrict__ ld,
+                                                      double* __restrict__ sd, double* __restrict__ g,
+                                                      double* __restrict__ lapd, float* __restrict__ dcond, long nmat,
+                                                      int n, int C) {
   constexpr int CPP = 64 / NMAX;  // channels per pass
   extern __shared__ double dsm[];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -501,7 +569,7 @@ __global__ __launch_bounds__(64 * WPB) void k_det_lap(const float* __restrict__ 
   double* sMi = dsm + (size_t)wv * (NMAX * NMAX + 64 * NMAX);
   double* sY = sMi + NMAX * NMAX;
   const long nn = (long)n * n;
-  const float* Ab = A + bd * C * nn;
+  const TA* Ab = A + bd * C * nn;
   const bool act = lane < n;
   const int nd = 3 * n;
 
@@ -560,7 +628,7 @@ __global__ __launch_bounds__(64 * WPB) void k_det_lap(const float* __restrict__ 
   for (int k0 = 0; k0 < nd; k0 += CPP) {
     const int k = k0 + cg;
     const bool kact = k < nd;
-    const float* Tg = Ab + (long)(1 + (kact ? k : 0)) * nn;
+    const TA* Tg = Ab + (long)(1 + (kact ? k : 0)) * nn;
     double jc[NMAX];
 #pragma unroll
     for (int e = 0; e < NMAX; ++e) jc[e] = (cact && kact && e < n) ? (double)Tg[e * n + c] : 0.0;
@@ -594,7 +662,7 @@ __global__ __launch_bounds__(64 * WPB) void k_det_lap(const float* __restrict__ 
       diag += __shfl_xor(diag, o, NMAX);
     }
     if (c == 0 && kact) {
-      if (valid_mat) g[bd * nd + k] = (float)diag;
+      if (valid_mat) g[bd * nd + k] = diag;
       qacc += qp;
     }
     __syncthreads();
@@ -603,7 +671,7 @@ __global__ __launch_bounds__(64 * WPB) void k_det_lap(const float* __restrict__ 
 #pragma unroll
   for (int o = NMAX; o < 64; o <<= 1) qacc += __shfl_xor(qacc, o, 64);
   // tr(M^-1 Lap M): lane c takes column c of Lap M and row c of M^-1
-  const float* Tl = Ab + (long)(C - 1) * nn;
+  const TA* Tl = Ab + (long)(C - 1) * nn;
   double tp = 0.0;
   if (act) {
 #pragma unroll
@@ -612,45 +680,48 @@ __global__ __launch_bounds__(64 * WPB) void k_det_lap(const float* __restrict__ 
   }
   tp = wave_sum_d(tp);
   if (lane == 0 && valid_mat) {
-    ld[bd] = (float)logabs;
-    sd[bd] = (float)sgn;
-    lapd[bd] = (float)(tp - qacc);
-    dflags[bd] = (cond_thr > 0.f && !(normA * normMi <= (double)cond_thr)) ? 1u : 0u;
+    ld[bd] = logabs;
+    sd[bd] = sgn;
+    lapd[bd] = tp - qacc;
+    dcond[bd] = (float)(normA * normMi);
   }
 }
 
-void launch_det(const float* A, float* ld, float* sd, float* g, float* lapd, uint32_t* dflags, float cond_thr,
-                const Shape& sh, hipStream_t s) {
+template <typename TA>
+void launch_det(const TA* A, double* ld, double* sd, double* g, double* lapd, float* dcond, const Shape& sh,
+                hipStream_t s) {
   const int n = sh.n;
   const long nmat = (long)sh.B * sh.D;
   const long stride = (long)sh.C * n * n;
   if (sh.C == 1) {
     const unsigned grid = (unsigned)((nmat + 3) / 4);
     if (n <= 16)
-      hipLaunchKernelGGL(k_det_value<16>, dim3(grid), dim3(256), 0, s, A, ld, sd, dflags, cond_thr, nmat, n, stride);
+      hipLaunchKernelGGL((k_det_value<16, TA>), dim3(grid), dim3(256), 0, s, A, ld, sd, dcond, nmat, n, stride);
     else if (n <= 32)
-      hipLaunchKernelGGL(k_det_value<32>, dim3(grid), dim3(256), 0, s, A, ld, sd, dflags, cond_thr, nmat, n, stride);
+      hipLaunchKernelGGL((k_det_value<32, TA>), dim3(grid), dim3(256), 0, s, A, ld, sd, dcond, nmat, n, stride);
     else
-      hipLaunchKernelGGL(k_det_value<64>, dim3(grid), dim3(256), 0, s, A, ld, sd, dflags, cond_thr, nmat, n, stride);
+      hipLaunchKernelGGL((k_det_value<64, TA>), dim3(grid), dim3(256), 0, s, A, ld, sd, dcond, nmat, n, stride);
     return;
   }
   if (n <= 16) {
     constexpr int NM = 16, W = 4;
     const size_t shm = (size_t)W * (NM * NM + 64 * NM) * sizeof(double);
-    hipLaunchKernelGGL((k_det_lap<NM, W>), dim3((unsigned)((nmat + W - 1) / W)), dim3(64 * W), shm, s, A, ld, sd, g,
-                       lapd, dflags, cond_thr, nmat, n, sh.C);
+    hipLaunchKernelGGL((k_det_lap<NM, W, TA>), dim3((unsigned)((nmat + W - 1) / W)), dim3(64 * W), shm, s, A, ld, sd,
+                       g, lapd, dcond, nmat, n, sh.C);
   } else if (n <= 32) {
     constexpr int NM = 32, W = 2;
     const size_t shm = (size_t)W * (NM * NM + 64 * NM) * sizeof(double);
-    hipLaunchKernelGGL((k_det_lap<NM, W>), dim3((unsigned)((nmat + W - 1) / W)), dim3(64 * W), shm, s, A, ld, sd, g,
-                       lapd, dflags, cond_thr, nmat, n, sh.C);
+    hipLaunchKernelGGL((k_det_lap<NM, W, TA>), dim3((unsigned)((nmat + W - 1) / W)), dim3(64 * W), shm, s, A, ld, sd,
+                       g, lapd, dcond, nmat, n, sh.C);
   } else {
     constexpr int NM = 64, W = 1;
     const size_t shm = (size_t)W * (NM * NM + 64 * NM) * sizeof(double);
-    hipLaunchKernelGGL((k_det_lap<NM, W>), dim3((unsigned)nmat), dim3(64 * W), shm, s, A, ld, sd, g, lapd, dflags,
-                       cond_thr, nmat, n, sh.C);
+    hipLaunchKernelGGL((k_det_lap<NM, W, TA>), dim3((unsigned)nmat), dim3(64 * W), shm, s, A, ld, sd, g, lapd, dcond,
+                       nmat, n, sh.C);
   }
 }
+template void launch_det<float>(const float*, double*, double*, double*, double*, float*, const Shape&, hipStream_t);
+template void launch_det<double>(const double*, double*, double*, double*, double*, float*, const Shape&, hipStream_t);
 
 // ------------------------------------------------------------------------------------------------
 // Per-walker combination (FP64).  Sum of determinants with max shift and LOG_EPSILON
@@ -664,26 +735,32 @@ void launch_det(const float* A, float* ld, float* sd, float* g, float* lapd, uin
 // hamiltonian.py:17-39.
 // ------------------------------------------------------------------------------------------------
 template <bool LAP>
-__global__ void k_combine(const float* __restrict__ ld, const float* __restrict__ sd, const float* __restrict__ g,
-                          const float* __restrict__ lapd, const uint32_t* __restrict__ dflags,
+__global__ void k_combine(const double* __restrict__ ld, const double* __restrict__ sd, const double* __restrict__ g,
+                          const double* __restrict__ lapd, const float* __restrict__ dcond, float cond_thr,
                           const float* __restrict__ r, const float* __restrict__ R, const float* __restrict__ Z,
                           float* __restrict__ out_lp, float* __restrict__ out_phase, float* __restrict__ out_eloc,
-                          uint32_t* __restrict__ out_flags, Shape sh) {
+                          uint32_t* __restrict__ out_flags, float* __restrict__ out_kappa, Shape sh) {
   const long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= sh.B) return;
   const int D = sh.D, n = sh.n, nd = 3 * n;
-  const float* l = ld + b * D;
-  const float* sg = sd + b * D;
+  const double* l = ld + b * D;
+  const double* sg = sd + b * D;
   double m = -INFINITY;
   for (int d = 0; d < D; ++d) m = fmax(m, (double)l[d]);
-  double S = 0.0;
+  double S = 0.0, Sabs = 0.0, cmax = 0.0;
   int cnt = 0;
   uint32_t fl = 0;
   for (int d = 0; d < D; ++d) {
-    S += (double)sg[d] * exp((double)l[d] - m);
-    if ((double)l[d] == m) ++cnt;
-    if (dflags[b * D + d]) fl |= DWF_FP64;
+    const double e = sg[d] * exp(l[d] - m);
+    S += e;
+    Sabs += fabs(e);
+    if (l[d] == m) ++cnt;
+    cmax = fmax(cmax, (double)dcond[b * D + d]);
   }
+  // conditioning of log|psi|: matrix condition times the cancellation of the determinant sum
+  const double kappa = cmax * Sabs / fabs(S);
+  if (out_kappa) out_kappa[b] = (float)kappa;
+  if (cond_thr > 0.f && !(kappa <= (double)cond_thr)) fl |= DWF_FP64;
   const double eps = 1e-8;
   const double aS = fabs(S);
   const double fval = log(aS + eps) + m;
@@ -694,7 +771,7 @@ __global__ void k_combine(const float* __restrict__ ld, const float* __restrict_
   if (LAP) {
     const double sgnS = (S > 0.0) ? 1.0 : ((S < 0.0) ? -1.0 : 0.0);
     const double den = aS + eps;
-    const float* gb = g + b * D * nd;
+    const double* gb = g + b * D * nd;
     double lapm = 0.0;
     for (int d = 0; d < D; ++d)
       if ((double)l[d] == m) lapm += lapd[b * D + d];
@@ -748,16 +825,16 @@ __global__ void k_combine(const float* __restrict__ ld, const float* __restrict_
   if (out_flags) out_flags[b] = fl;
 }
 
-void launch_combine(const float* ld, const float* sd, const float* g, const float* lapd, const uint32_t* dflags,
-                    const float* r, const float* R, const float* Z, float* out_lp, float* out_phase, float* out_eloc,
-                    uint32_t* out_flags, const Shape& sh, hipStream_t s) {
+void launch_combine(const double* ld, const double* sd, const double* g, const double* lapd, const float* dcond,
+                    float cond_thr, const float* r, const float* R, const float* Z, float* out_lp, float* out_phase,
+                    float* out_eloc, uint32_t* out_flags, float* out_kappa, const Shape& sh, hipStream_t s) {
   const unsigned grid = (unsigned)((sh.B + 127) / 128);
   if (sh.C > 1)
-    hipLaunchKernelGGL(k_combine<true>, dim3(grid), dim3(128), 0, s, ld, sd, g, lapd, dflags, r, R, Z, out_lp,
-                       out_phase, out_eloc, out_flags, sh);
+    hipLaunchKernelGGL(k_combine<true>, dim3(grid), dim3(128), 0, s, ld, sd, g, lapd, dcond, cond_thr, r, R, Z, out_lp,
+                       out_phase, out_eloc, out_flags, out_kappa, sh);
   else
-    hipLaunchKernelGGL(k_combine<false>, dim3(grid), dim3(128), 0, s, ld, sd, g, lapd, dflags, r, R, Z, out_lp,
-                       out_phase, out_eloc, out_flags, sh);
+    hipLaunchKernelGGL(k_combine<false>, dim3(grid), dim3(128), 0, s, ld, sd, g, lapd, dcond, cond_thr, r, R, Z,
+                       out_lp, out_phase, out_eloc, out_flags, out_kappa, sh);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -776,6 +853,28 @@ __global__ void k_fill_i32(int32_t* p, int32_t v, int count) {
 }
 void launch_fill_i32(int32_t* p, int32_t v, int count, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_i32, dim3((count + 255) / 256), dim3(256), 0, s, p, v, count);
+}
+
+// Compaction of flagged walkers for the FP64 recompute: out[k] = r[idx[k]]; and the inverse scatter.
+__global__ void k_gather_walkers(const float* __restrict__ r, const int32_t* __restrict__ idx, int count, int n,
+                                 float* __restrict__ out) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long per = 3L * n;
+  if (t >= count * per) return;
+  const long k = t / per, c = t % per;
+  out[t] = r[(long)idx[k] * per + c];
+}
+void launch_gather_walkers(const float* r, const int32_t* idx, int count, int n, float* out, hipStream_t s) {
+  const long total = (long)count * 3 * n;
+  hipLaunchKernelGGL(k_gather_walkers, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, r, idx, count, n, out);
+}
+__global__ void k_scatter_values(const float* __restrict__ v, const int32_t* __restrict__ idx, int count,
+                                 float* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) out[idx[t]] = v[t];
+}
+void launch_scatter_values(const float* v, const int32_t* idx, int count, float* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_scatter_values, dim3((count + 255) / 256), dim3(256), 0, s, v, idx, count, out);
 }
 
 // r' = r + normal * stepsize

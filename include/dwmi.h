@@ -33,16 +33,19 @@ typedef int dw_status;
 typedef struct dw_model dw_model;
 typedef void* dw_stream; /* a hipStream_t; NULL is the default stream */
 
-/* Precision forms.  DW_FP32: every GEMM in FP32 on MFMA.  DW_BF16X3: embedding and orbital GEMMs on
- * BF16 MFMA with FP32 accumulation, operands split into hi+lo BF16 pairs (three products).
- * DW_BF16: one BF16 product (fastest, least accurate). */
+/* Precision forms.  DW_FP32: every GEMM in FP32 on MFMA, determinant stage in FP64.  DW_BF16X3:
+ * embedding and orbital GEMMs on BF16 MFMA with FP32 accumulation, operands split into hi+lo BF16 pairs
+ * (three products).  DW_BF16: one BF16 product.  DW_FP64: the whole pass in FP64 (the reference's
+ * float_precision="float64" mode, configuration.py:1893).  With fp64_cond_threshold > 0, walkers whose
+ * conditioning estimate exceeds it are recomputed by dw_local_energy in FP64 and flagged DW_FLAG_FP64_DET. */
 #define DW_FP32 0
 #define DW_BF16X3 1
 #define DW_BF16 2
+#define DW_FP64 3
 
 /* Per-walker flag bits written to out_flags. */
 #define DW_FLAG_NONFINITE 1u     /* log|psi|^2 or E_loc is NaN/inf                          */
-#define DW_FLAG_FP64_DET 2u      /* at least one determinant was refactorised in FP64       */
+#define DW_FLAG_FP64_DET 2u      /* conditioning above threshold: recomputed in FP64         */
 #define DW_FLAG_FORCED_ACCEPT 4u /* MCMC: accepted because walker age reached max_age       */
 
 typedef struct {
@@ -53,8 +56,8 @@ typedef struct {
   int32_t width1;  /* 1-el stream width (256)                    */
   int32_t width2;  /* 2-el stream width (32)                     */
   int32_t n_iter;  /* FermiNet iterations (4)                    */
-  int32_t precision;       /* DW_FP32 / DW_BF16X3 / DW_BF16       */
-  float fp64_cond_threshold; /* condition estimate above which a det is redone in FP64 (<=0: never) */
+  int32_t precision;       /* DW_FP32 / DW_BF16X3 / DW_BF16 / DW_FP64 */
+  float fp64_cond_threshold; /* walker conditioning above which dw_local_energy recomputes in FP64 (<=0: never) */
 } dw_model_desc;
 
 /* Create a model for one (n_el, n_up, n_ion) shape.  Replaces build_log_psi_squared's model
@@ -101,6 +104,11 @@ dw_status dw_mcmc_steps(dw_model* m, float* r, int32_t n_walkers, const float* R
                         int32_t n_steps, dw_chain_scalars* chain, int32_t max_age,
                         int32_t stepsize_update_interval, float target_acceptance, float min_stepsize,
                         float max_stepsize, float* out_acc, uint32_t* out_flags, dw_stream stream);
+
+/* Copy the per-walker conditioning estimate of the last pass into out (device, n_walkers floats, ordered on
+ * `stream`): max_d |A_d|_1 |A_d^-1|_1 (Laplacian pass; pivot ratio in the value pass) times
+ * sum_d |s_d e^{l_d-m}| / |sum_d s_d e^{l_d-m}|.  For dw_local_energy with a chunk cap, the last chunk. */
+dw_status dw_model_conditioning(dw_model* m, float* out, int32_t n_walkers, dw_stream stream);
 
 /* Bytes of device workspace the model holds after sizing for n_walkers in the given mode
  * (0 = log|psi| / MCMC, 1 = local energy). */

@@ -127,3 +127,59 @@ def test_mcmc_replay_matches_oracle(cuda):
     np.testing.assert_allclose(acc.cpu().numpy(), accs, atol=3.0 / B)
     s, a, n = read_chain(chain)
     assert n == S and abs(s - 0.02) < 1e-7
+
+
+@pytest.mark.parametrize("name,B", [("lih", 16), ("n2", 4)])
+def test_fp64_form_matches_oracle(cuda, name, B):
+    """The whole pass in FP64 (the reference's float64 mode): only the float32 outputs round."""
+    shape, params, R, Z, mapping = system(name)
+    r = walkers(R, mapping, B, seed=4)
+    m = _native(shape, params, "fp64")
+    e = torch.empty(B, device="cuda")
+    lp = torch.empty(B, device="cuda")
+    lp2 = torch.empty(B, device="cuda")
+    m.local_energy(_dev(r), _dev(R), _dev(Z), e, lp)
+    m.logpsi(_dev(r), _dev(R), _dev(Z), lp2)
+    torch.cuda.synchronize()
+    e_ref = oh.local_energy(params, shape.n_up, shape.n_dn, r.astype(np.float64), R, Z).numpy()
+    lp_ref = _oracle_lp(params, shape, r, R)[1].numpy()
+    kappa = owf.conditioning(params, shape.n_up, shape.n_dn, r, R)
+    de = np.abs(e.cpu().numpy() - e_ref)
+    dl = np.abs(lp2.cpu().numpy() - lp_ref)
+    print(f"{name} fp64: max |dE| {de.max():.2e}, max |dlp| {dl.max():.2e}")
+    u64 = 2.0 ** -53
+    assert np.all(de <= (1 + np.abs(e_ref)) * (2e-7 + 64 * u64 * kappa))
+    assert np.all(dl <= 2e-5 * np.maximum(1, np.abs(lp_ref) / 10) + 64 * u64 * kappa)
+    assert np.all(np.abs(lp.cpu().numpy() - lp_ref) <= 2e-5 * np.maximum(1, np.abs(lp_ref) / 10) + 64 * u64 * kappa)
+
+
+def test_flagged_walkers_are_recomputed_in_fp64(cuda):
+    """FP32 form with a conditioning threshold: walkers above it come back from the FP64 pass."""
+    from deeperwin_amd._native import FLAG_FP64_DET, NativeModel
+
+    shape, params, R, Z, mapping = system("n2")
+    B = 16
+    r = walkers(R, mapping, B, seed=5)
+    plain = _native(shape, params, "fp32")
+    mixed = NativeModel(shape, "fp32", fp64_cond_threshold=1e5)
+    mixed.set_params(params)
+    e1 = torch.empty(B, device="cuda")
+    e2 = torch.empty(B, device="cuda")
+    fl = torch.empty(B, device="cuda", dtype=torch.int32)
+    plain.local_energy(_dev(r), _dev(R), _dev(Z), e1)
+    mixed.local_energy(_dev(r), _dev(R), _dev(Z), e2, None, fl)
+    kap = mixed.conditioning(B)
+    torch.cuda.synchronize()
+    e_ref = oh.local_energy(params, shape.n_up, shape.n_dn, r.astype(np.float64), R, Z).numpy()
+    flagged = (fl.cpu().numpy() & FLAG_FP64_DET) != 0
+    kappa_ref = owf.conditioning(params, shape.n_up, shape.n_dn, r, R)
+    print(f"flagged {flagged.sum()} / {B}; device kappa {kap.cpu().numpy()[:4]} vs oracle {kappa_ref[:4]}")
+    assert flagged.any()
+    d1 = np.abs(e1.cpu().numpy() - e_ref)
+    d2 = np.abs(e2.cpu().numpy() - e_ref)
+    # recomputed walkers are FP64-accurate; the others are bit-identical to the plain FP32 form
+    assert np.all(d2[flagged] <= 1e-5 * (1 + np.abs(e_ref[flagged])))
+    np.testing.assert_array_equal(e2.cpu().numpy()[~flagged], e1.cpu().numpy()[~flagged])
+    # the device estimate tracks the FP64 conditioning (1-norm vs 2-norm condition: within a factor ~n)
+    k = kap.cpu().numpy()
+    assert np.all(k >= kappa_ref / 20) and np.all(k <= kappa_ref * 20)
