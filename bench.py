@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3", "bf16"])
     ap.add_argument("--walkers", type=int, default=2048)
     ap.add_argument("--mcmc-steps", type=int, default=20)
-    ap.add_argument("--cpu-walkers", type=int, default=32)
+    ap.add_argument("--cpu-walkers", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     return ap.parse_args()

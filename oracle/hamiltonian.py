@@ -46,8 +46,9 @@ def kinetic_energy(params: Dict, n_up: int, n_dn: int, r: torch.Tensor, R: torch
     n = n_up + n_dn
     R = R.to(dtype)
 
-    def f(x):  # x: [3n] -> log psi^2 of one walker
-        return wf.log_psi_sqr(params, n_up, n_dn, x.reshape(n, 3), R, dtype=dtype)[1]
+    def f(x):  # x: [3n] -> log psi^2 of one walker (evaluated as a batch of one: in this PyTorch build the
+        # float32 forward-over-reverse of slogdet fails for un-batched inputs)
+        return wf.log_psi_sqr(params, n_up, n_dn, x.reshape(1, n, 3), R, dtype=dtype)[1][0]
 
     grad_f = torch.func.grad(f)
     eye = torch.eye(3 * n, dtype=dtype)

@@ -8,7 +8,7 @@ FP64 by ``oracle.wavefunction.conditioning``).  Per walker:
   log|psi|^2 : |d| <= 5e-5 max(1, |ref| / 10) + 4 u kappa
   E_loc      : |d| <= (1 + |ref|) (1e-4 + 4 u kappa)
 and over the batch (catches errors that every walker shares):
-  median |d log|psi|^2| <= 5e-4,   median |d E_loc| <= 3e-3
+  median |d log|psi|^2| <= 5e-4,   median |d E_loc| <= 1e-4 (1 + median |E_loc|)
 MCMC replay: with the same supplied draws >= 98% of walkers follow the oracle's chain exactly (a walker
 can only leave it at a decision within the FP32 noise of the threshold).
 """
@@ -71,7 +71,7 @@ def test_logpsi_matches_oracle(cuda, name, B):
     np.testing.assert_array_equal(ph.cpu().numpy(), ph_ref.numpy().astype(np.float32))
 
 
-@pytest.mark.parametrize("name,B", [("lih", 32), ("n2", 8), ("c4h2", 2)])
+@pytest.mark.parametrize("name,B", [("lih", 32), ("n2", 8), ("c4h2", 4)])
 def test_local_energy_matches_oracle(cuda, name, B):
     shape, params, R, Z, mapping = system(name)
     r = walkers(R, mapping, B, seed=2)
@@ -87,7 +87,7 @@ def test_local_energy_matches_oracle(cuda, name, B):
     print(f"{name}: E_loc ref {e_ref[:4]} max |dE| = {err.max():.3e}, median {np.median(err):.2e}, "
           f"max err/tol {np.max(err / _tol_e(e_ref, kappa)):.3f}")
     assert np.all(err <= _tol_e(e_ref, kappa))
-    assert np.median(err) <= 3e-3
+    assert np.median(err) <= 1e-4 * (1 + np.median(np.abs(e_ref)))
     lp_ref = _oracle_lp(params, shape, r, R)[1].numpy()
     assert np.all(np.abs(lp.cpu().numpy() - lp_ref) <= _tol_lp(lp_ref, kappa))
     assert int(fl.cpu().numpy().max()) == 0
