@@ -52,13 +52,17 @@ __device__ __forceinline__ int tile_index(int bid, int total) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// FP32 MFMA
+// FP32 MFMA.  Tile (64 WM) x (64 WN) per 256-thread workgroup; 4 waves in 2 x 2, each wave owns
+// WM x WN blocks of 32 x 32 (v_mfma_f32_32x32x2_f32).  WM = WN = 2 for large GEMMs; smaller tiles give
+// enough workgroups for the per-walker and orbital GEMMs of a value pass.
 // ------------------------------------------------------------------------------------------------
-template <int FLAGS>
+template <int FLAGS, int WM, int WN>
 __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ A, int lda, const float* __restrict__ Bt,
                                                   int ldb, float* __restrict__ Cm, int ldc, int M, int N, int K,
                                                   int tilesN, int tilesTotal, GemmEpi ep) {
-  constexpr int BM = 128, BN = 128, BK = 16, PAD = 4;
+  constexpr int BM = 64 * WM, BN = 64 * WN, BK = 16, PAD = 4;
+  constexpr int AV = BM * BK / 4, BV = BN * BK / 4;  // float4 per tile
+  constexpr int AIT = (AV + 255) / 256, BIT = (BV + 255) / 256;
   __shared__ float As[BK][BM + PAD];
   __shared__ float Bs[BK][BN + PAD];
   const int tile = tile_index(blockIdx.x, tilesTotal);
@@ -68,37 +72,57 @@ __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ A, i
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int wm = wid >> 1, wn = wid & 1;
 
-  v16f acc[2][2];
+  v16f acc[WM][WN];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < WM; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < WN; ++b)
 #pragma unroll
       for (int j = 0; j < 16; ++j) acc[a][b][j] = 0.f;
 
-  float4 ra[2], rb[2];
+  float4 ra[AIT], rb[BIT];
   auto gload = [&](int k0) {
 #pragma unroll
-    for (int it = 0; it < 2; ++it) {
-      const int idx = tid + 256 * it, m = idx >> 2, kq = (idx & 3) * 4;
-      long gr = row0 + m;
-      if (gr > M - 1) gr = M - 1;
-      ra[it] = *(const float4*)(A + gr * lda + k0 + kq);
-      rb[it] = *(const float4*)(Bt + (long)(col0 + m) * ldb + k0 + kq);
+    for (int it = 0; it < AIT; ++it) {
+      const int idx = tid + 256 * it;
+      if (idx < AV) {
+        const int m = idx >> 2, kq = (idx & 3) * 4;
+        long gr = row0 + m;
+        if (gr > M - 1) gr = M - 1;
+        ra[it] = *(const float4*)(A + gr * lda + k0 + kq);
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < BIT; ++it) {
+      const int idx = tid + 256 * it;
+      if (idx < BV) {
+        const int m = idx >> 2, kq = (idx & 3) * 4;
+        rb[it] = *(const float4*)(Bt + (long)(col0 + m) * ldb + k0 + kq);
+      }
     }
   };
   auto sstore = [&]() {
 #pragma unroll
-    for (int it = 0; it < 2; ++it) {
-      const int idx = tid + 256 * it, m = idx >> 2, kq = (idx & 3) * 4;
-      As[kq + 0][m] = ra[it].x;
-      As[kq + 1][m] = ra[it].y;
-      As[kq + 2][m] = ra[it].z;
-      As[kq + 3][m] = ra[it].w;
-      Bs[kq + 0][m] = rb[it].x;
-      Bs[kq + 1][m] = rb[it].y;
-      Bs[kq + 2][m] = rb[it].z;
-      Bs[kq + 3][m] = rb[it].w;
+    for (int it = 0; it < AIT; ++it) {
+      const int idx = tid + 256 * it;
+      if (idx < AV) {
+        const int m = idx >> 2, kq = (idx & 3) * 4;
+        As[kq + 0][m] = ra[it].x;
+        As[kq + 1][m] = ra[it].y;
+        As[kq + 2][m] = ra[it].z;
+        As[kq + 3][m] = ra[it].w;
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < BIT; ++it) {
+      const int idx = tid + 256 * it;
+      if (idx < BV) {
+        const int m = idx >> 2, kq = (idx & 3) * 4;
+        Bs[kq + 0][m] = rb[it].x;
+        Bs[kq + 1][m] = rb[it].y;
+        Bs[kq + 2][m] = rb[it].z;
+        Bs[kq + 3][m] = rb[it].w;
+      }
     }
   };
 
@@ -111,25 +135,26 @@ __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ A, i
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 2) {
       const int kr = kk + (lane >> 5);
-      const float a0 = As[kr][wm * 64 + (lane & 31)];
-      const float a1 = As[kr][wm * 64 + 32 + (lane & 31)];
-      const float b0 = Bs[kr][wn * 64 + (lane & 31)];
-      const float b1 = Bs[kr][wn * 64 + 32 + (lane & 31)];
-      acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[1][1], 0, 0, 0);
+      float a[WM], b[WN];
+#pragma unroll
+      for (int i = 0; i < WM; ++i) a[i] = As[kr][wm * 32 * WM + i * 32 + (lane & 31)];
+#pragma unroll
+      for (int j = 0; j < WN; ++j) b[j] = Bs[kr][wn * 32 * WN + j * 32 + (lane & 31)];
+#pragma unroll
+      for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int j = 0; j < WN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], b[j], acc[i][j], 0, 0, 0);
     }
   }
 
 #pragma unroll
-  for (int mi = 0; mi < 2; ++mi)
+  for (int mi = 0; mi < WM; ++mi)
 #pragma unroll
-    for (int ni = 0; ni < 2; ++ni)
+    for (int ni = 0; ni < WN; ++ni)
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
-        const long r = row0 + wm * 64 + mi * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-        const int c = col0 + wn * 64 + ni * 32 + (lane & 31);
+        const long r = row0 + wm * 32 * WM + mi * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+        const int c = col0 + wn * 32 * WN + ni * 32 + (lane & 31);
         if (r < M && c < N) epi_store<FLAGS, float>(acc[mi][ni][j], r, c, Cm, ldc, ep);
       }
 }
@@ -322,17 +347,42 @@ void gemm_f64(const double* A, int lda, const double* Bt, int ldb, double* C, in
                      ldb, C, ldc, M, N, K, tilesN, total, ep);
 }
 
+template <int FLAGS, int WM, int WN>
+static void launch_f32(const float* A, int lda, const float* Bt, int ldb, float* C, int ldc, int M, int N, int K,
+                       const GemmEpi& ep, hipStream_t s) {
+  const int tilesM = (M + 64 * WM - 1) / (64 * WM), tilesN = (N + 64 * WN - 1) / (64 * WN);
+  const int total = tilesM * tilesN;
+  const int grid = (total + 7) / 8 * 8;
+  hipLaunchKernelGGL((k_gemm_f32<FLAGS, WM, WN>), dim3(grid), dim3(256), 0, s, A, lda, Bt, ldb, C, ldc, M, N, K,
+                     tilesN, total, ep);
+}
+
+// Workgroups needed to fill the chip a few times over: the largest tile that still gives >= 4 per CU.
+static int pick_tile(long M, int N) {
+  const long wg128 = ((M + 127) / 128) * ((N + 127) / 128);
+  if (wg128 >= 1024) return 2;
+  const long wg64x128 = ((M + 63) / 64) * ((N + 127) / 128);
+  if (wg64x128 >= 1024) return 1;
+  return 0;
+}
+
 template <int FLAGS>
 static void gemm_dispatch(GemmPrec prec, const float* A, int lda, const float* Bt, const uint16_t* Bhi,
                           const uint16_t* Blo, int ldb, float* C, int ldc, int M, int N, int K, const GemmEpi& ep,
                           hipStream_t s) {
+  if (prec == GEMM_F32) {
+    // Bt holds pad128(N) rows, so every tile shape reads inside it
+    switch (pick_tile(M, N)) {
+      case 2: launch_f32<FLAGS, 2, 2>(A, lda, Bt, ldb, C, ldc, M, N, K, ep, s); break;
+      case 1: launch_f32<FLAGS, 1, 2>(A, lda, Bt, ldb, C, ldc, M, N, K, ep, s); break;
+      default: launch_f32<FLAGS, 1, 1>(A, lda, Bt, ldb, C, ldc, M, N, K, ep, s); break;
+    }
+    return;
+  }
   const int tilesM = (M + 127) / 128, tilesN = (N + 127) / 128;
   const int total = tilesM * tilesN;
   const int grid = (total + 7) / 8 * 8;
-  if (prec == GEMM_F32)
-    hipLaunchKernelGGL((k_gemm_f32<FLAGS>), dim3(grid), dim3(256), 0, s, A, lda, Bt, ldb, C, ldc, M, N, K, tilesN,
-                       total, ep);
-  else if (prec == GEMM_BF16X3)
+  if (prec == GEMM_BF16X3)
     hipLaunchKernelGGL((k_gemm_bf16<FLAGS, 1>), dim3(grid), dim3(256), 0, s, A, lda, Bhi, Blo, ldb, C, ldc, M, N, K,
                        tilesN, total, ep);
   else

@@ -526,30 +526,84 @@ __device__ __forceinline__ void finish_slogdet(double mydiag, int swaps, bool si
   }
 }
 
-// Value mode: 4 matrices per 256-thread workgroup, no LDS.
+// Value mode: 64 / NMAX matrices per wavefront (lane group g holds matrix g, lane j of the group holds
+// column j), 4 wavefronts per workgroup, no LDS.  Pivot search in the pivot column's lane, the pivot
+// index, pivot and multipliers are broadcast inside the group with shuffles.
 template <int NMAX, typename TA>
 __global__ __launch_bounds__(256) void k_det_value(const TA* __restrict__ A, double* __restrict__ ld,
                                                    double* __restrict__ sd, float* __restrict__ dcond, long nmat,
                                                    int n, long stride) {
+  constexpr int G = 64 / NMAX;
   const int lane = threadIdx.x & 63;
-  const long bd = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (bd >= nmat) return;
-  const TA* Ab = A + bd * stride;
-  const bool act = lane < n;
+  const int g = lane / NMAX, j = lane % NMAX, base = g * NMAX;
+  const long bd = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * G + g;
+  const bool valid = bd < nmat;
+  const TA* Ab = A + (valid ? bd : nmat - 1) * stride;  // out-of-range groups factor a copy, never stored
+  const bool act = j < n;
   double col[NMAX];
-  int perm[NMAX];
 #pragma unroll
-  for (int i = 0; i < NMAX; ++i) col[i] = (act && i < n) ? (double)Ab[i * n + lane] : 0.0;
-  double mydiag = 1.0;
+  for (int i = 0; i < NMAX; ++i) col[i] = (act && i < n) ? (double)Ab[i * n + j] : 0.0;
+  int swaps = 0;
   bool sing = false;
-  const int swaps = lu_columns<NMAX>(col, perm, mydiag, sing, n, lane);
-  double logabs, sgn;
-  finish_slogdet<NMAX>(mydiag, swaps, sing, n, lane, logabs, sgn);
-  const double umax = wave_max_d(act ? fabs(mydiag) : 0.0);
-  const double umin = wave_min_d(act ? fabs(mydiag) : 1e300);
-  if (lane == 0) {
-    ld[bd] = logabs;
-    sd[bd] = sgn;
+  double mydiag = 1.0;
+#pragma unroll
+  for (int p = 0; p < NMAX; ++p) {
+    if (p < n) {
+      double best = -1.0;
+      int bidx = p;
+#pragma unroll
+      for (int i = p; i < NMAX; ++i) {
+        if (i < n) {
+          const double a = fabs(col[i]);
+          if (a > best) {
+            best = a;
+            bidx = i;
+          }
+        }
+      }
+      const int piv = __shfl(bidx, base + p, 64);
+      const double bval = __shfl(best, base + p, 64);
+      if (!(bval > 0.0)) sing = true;
+      if (piv != p) {
+        ++swaps;
+#pragma unroll
+        for (int i = p + 1; i < NMAX; ++i) {
+          if (i == piv) {
+            const double t = col[p];
+            col[p] = col[i];
+            col[i] = t;
+          }
+        }
+      }
+      const double pivot = __shfl(col[p], base + p, 64);
+      if (j == p) {
+        mydiag = col[p];
+#pragma unroll
+        for (int i = p + 1; i < NMAX; ++i)
+          if (i < n) col[i] = col[i] / pivot;
+      }
+#pragma unroll
+      for (int i = p + 1; i < NMAX; ++i) {
+        if (i < n) {
+          const double li = __shfl(col[i], base + p, 64);
+          if (j > p) col[i] -= li * col[p];
+        }
+      }
+    }
+  }
+  double lg = act ? log(fabs(mydiag)) : 0.0;
+  int neg = (act && mydiag < 0.0) ? 1 : 0;
+  double umax = act ? fabs(mydiag) : 0.0, umin = act ? fabs(mydiag) : 1e300;
+#pragma unroll
+  for (int o = NMAX / 2; o > 0; o >>= 1) {
+    lg += __shfl_xor(lg, o, NMAX);
+    neg += __shfl_xor(neg, o, NMAX);
+    umax = fmax(umax, __shfl_xor(umax, o, NMAX));
+    umin = fmin(umin, __shfl_xor(umin, o, NMAX));
+  }
+  if (j == 0 && valid) {
+    ld[bd] = sing ? -INFINITY : lg;
+    sd[bd] = sing ? 0.0 : (((neg + swaps) & 1) ? -1.0 : 1.0);
     dcond[bd] = (float)(umax / umin);
   }
 }
@@ -687,20 +741,253 @@ __global__ __launch_bounds__(64 * WPB) void k_det_lap(const TA* __restrict__ A, 
   }
 }
 
+// Large matrices (n > 32): one 256-thread workgroup per matrix, FP64 in LDS.  Element-parallel
+// partial-pivot LU (wave 0 searches the pivot), inverse by per-column substitution, and for every tangent
+// channel the pair (a, c) / (c, a) of Y = M^-1 dM_k is formed by one thread, so tr Y and tr Y^2 need no
+// stored Y.
+template <typename TA, bool LAP>
+__global__ __launch_bounds__(256) void k_det_lds(const TA* __restrict__ A, double* __restrict__ ld,
+                                                 double* __restrict__ sd, double* __restrict__ g,
+                                                 double* __restrict__ lapd, float* __restrict__ dcond, int n, int C) {
+  extern __shared__ double sm[];
+  const int P = n + 1;
+  double* T = sm;           // LU factors, then channel matrices
+  double* Mi = T + n * P;   // inverse
+  double* lcol = Mi + n * P;
+  double* red = lcol + 64;
+  int* perm = (int*)(red + 32);
+  __shared__ int piv_sh, sing_sh;
+  const long bd = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const long nn = (long)n * n;
+  const TA* Ab = A + bd * C * nn;
+  double csum_local = 0.0;
+  for (int e = tid; e < n * n; e += 256) T[(e / n) * P + (e % n)] = (double)Ab[e];
+  if (tid < n) perm[tid] = tid;
+  if (tid == 0) sing_sh = 0;
+  __syncthreads();
+  if (LAP) {  // 1-norm of M (max column sum)
+    for (int c = tid; c < n; c += 256) {
+      double cs = 0.0;
+      for (int rr = 0; rr < n; ++rr) cs += fabs(T[rr * P + c]);
+      csum_local = fmax(csum_local, cs);
+    }
+  }
+  int swaps = 0;
+  for (int p = 0; p < n; ++p) {
+    if (wid == 0) {
+      double best = -1.0;
+      int bidx = p;
+      const int rr = p + lane;
+      if (rr < n) {
+        best = fabs(T[rr * P + p]);
+        bidx = rr;
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ob = __shfl_xor(best, o, 64);
+        const int oi = __shfl_xor(bidx, o, 64);
+        if (ob > best || (ob == best && oi < bidx)) {
+          best = ob;
+          bidx = oi;
+        }
+      }
+      if (lane == 0) {
+        piv_sh = bidx;
+        if (!(best > 0.0)) sing_sh = 1;
+      }
+    }
+    __syncthreads();
+    const int piv = piv_sh;
+    if (piv != p) {
+      for (int j = tid; j < n; j += 256) {
+        const double t = T[p * P + j];
+        T[p * P + j] = T[piv * P + j];
+        T[piv * P + j] = t;
+      }
+      if (tid == 0) {
+        const int tp = perm[p];
+        perm[p] = perm[piv];
+        perm[piv] = tp;
+      }
+      ++swaps;
+    }
+    __syncthreads();
+    const double pivot = T[p * P + p];
+    for (int rr = p + 1 + tid; rr < n; rr += 256) {
+      const double l = T[rr * P + p] / pivot;
+      lcol[rr] = l;
+      T[rr * P + p] = l;
+    }
+    __syncthreads();
+    const int m = n - p - 1;
+    for (int e = tid; e < m * m; e += 256) {
+      const int rr = p + 1 + e / m, cc = p + 1 + e % m;
+      T[rr * P + cc] -= lcol[rr] * T[p * P + cc];
+    }
+    __syncthreads();
+  }
+  double lsum = 0.0;
+  int nneg = 0;
+  double umax = 0.0, umin = 1e300;
+  for (int q = tid; q < n; q += 256) {
+    const double up = T[q * P + q];
+    lsum += log(fabs(up));
+    nneg += up < 0.0 ? 1 : 0;
+    umax = fmax(umax, fabs(up));
+    umin = fmin(umin, fabs(up));
+  }
+  lsum = block_sum_d(lsum, red);
+  const double negs = block_sum_d((double)nneg, red + 8);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    umax = fmax(umax, __shfl_xor(umax, o, 64));
+    umin = fmin(umin, __shfl_xor(umin, o, 64));
+  }
+  __syncthreads();
+  if (lane == 0) {
+    lcol[wid] = umax;
+    lcol[8 + wid] = umin;
+  }
+  __syncthreads();
+  double gmax = 0.0, gmin = 1e300;
+  for (int w = 0; w < 4; ++w) {
+    gmax = fmax(gmax, lcol[w]);
+    gmin = fmin(gmin, lcol[8 + w]);
+  }
+  const bool sing = sing_sh != 0;
+  const double logabs = sing ? -INFINITY : lsum;
+  const double sgn = sing ? 0.0 : ((((int)negs + swaps) & 1) ? -1.0 : 1.0);
+  if (!LAP) {
+    if (tid == 0) {
+      ld[bd] = logabs;
+      sd[bd] = sgn;
+      dcond[bd] = (float)(gmax / gmin);
+    }
+    return;
+  }
+  __syncthreads();
+  // inverse: column j solves L U x = P e_j
+  for (int j = tid; j < n; j += 256) {
+    for (int p = 0; p < n; ++p) {
+      double y = (perm[p] == j) ? 1.0 : 0.0;
+      for (int q = 0; q < p; ++q) y -= T[p * P + q] * Mi[q * P + j];
+      Mi[p * P + j] = y;
+    }
+    for (int p = n - 1; p >= 0; --p) {
+      double x = Mi[p * P + j];
+      for (int q = p + 1; q < n; ++q) x -= T[p * P + q] * Mi[q * P + j];
+      Mi[p * P + j] = x / T[p * P + p];
+    }
+  }
+  __syncthreads();
+  double isum = 0.0;
+  for (int c = tid; c < n; c += 256) {
+    double cs = 0.0;
+    for (int rr = 0; rr < n; ++rr) cs += fabs(Mi[rr * P + c]);
+    isum = fmax(isum, cs);
+  }
+  double nA = 0.0, nI = 0.0;
+  {
+    double a1 = csum_local, a2 = isum;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      a1 = fmax(a1, __shfl_xor(a1, o, 64));
+      a2 = fmax(a2, __shfl_xor(a2, o, 64));
+    }
+    __syncthreads();
+    if (lane == 0) {
+      lcol[wid] = a1;
+      lcol[8 + wid] = a2;
+    }
+    __syncthreads();
+    for (int w = 0; w < 4; ++w) {
+      nA = fmax(nA, lcol[w]);
+      nI = fmax(nI, lcol[8 + w]);
+    }
+  }
+  const int nd = 3 * n;
+  const int npair = n * (n + 1) / 2;
+  double qacc = 0.0;
+  for (int ch = 1; ch <= nd; ++ch) {
+    __syncthreads();
+    const TA* Tg = Ab + (long)ch * nn;
+    for (int e = tid; e < n * n; e += 256) T[(e / n) * P + (e % n)] = (double)Tg[e];
+    __syncthreads();
+    double pd = 0.0, pq = 0.0;
+    for (int t = tid; t < npair; t += 256) {
+      // t -> (a, c) with a <= c
+      int a = 0, rem = t;
+      while (rem >= n - a) {
+        rem -= n - a;
+        ++a;
+      }
+      const int c = a + rem;
+      double xac = 0.0, xca = 0.0;
+      for (int q = 0; q < n; ++q) {
+        xac += Mi[a * P + q] * T[q * P + c];
+        xca += Mi[c * P + q] * T[q * P + a];
+      }
+      if (a == c) {
+        pd += xac;
+        pq += xac * xac;
+      } else {
+        pq += 2.0 * xac * xca;
+      }
+    }
+    const double gs = block_sum_d(pd, red);
+    const double qs = block_sum_d(pq, red + 8);
+    if (tid == 0) g[bd * nd + (ch - 1)] = gs;
+    qacc += qs;
+  }
+  __syncthreads();
+  const TA* Tl = Ab + (long)(C - 1) * nn;
+  double tp = 0.0;
+  for (int e = tid; e < n * n; e += 256) {
+    const int a = e / n, q = e % n;
+    tp += Mi[a * P + q] * (double)Tl[q * n + a];
+  }
+  const double tr = block_sum_d(tp, red);
+  if (tid == 0) {
+    ld[bd] = logabs;
+    sd[bd] = sgn;
+    lapd[bd] = tr - qacc;
+    dcond[bd] = (float)(nA * nI);
+  }
+}
+
 template <typename TA>
 void launch_det(const TA* A, double* ld, double* sd, double* g, double* lapd, float* dcond, const Shape& sh,
                 hipStream_t s) {
   const int n = sh.n;
   const long nmat = (long)sh.B * sh.D;
   const long stride = (long)sh.C * n * n;
+  if (n > 32) {
+    const size_t shm = (size_t)(2 * n * (n + 1) + 64 + 32) * sizeof(double) + 64 * sizeof(int);
+    if (sh.C == 1) {
+      if (shm > 65536) (void)hipFuncSetAttribute((const void*)k_det_lds<TA, false>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+      hipLaunchKernelGGL((k_det_lds<TA, false>), dim3((unsigned)nmat), dim3(256), shm, s, A, ld, sd, g, lapd, dcond, n,
+                         sh.C);
+    } else {
+      if (shm > 65536) (void)hipFuncSetAttribute((const void*)k_det_lds<TA, true>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+      hipLaunchKernelGGL((k_det_lds<TA, true>), dim3((unsigned)nmat), dim3(256), shm, s, A, ld, sd, g, lapd, dcond, n,
+                         sh.C);
+    }
+    return;
+  }
   if (sh.C == 1) {
-    const unsigned grid = (unsigned)((nmat + 3) / 4);
-    if (n <= 16)
-      hipLaunchKernelGGL((k_det_value<16, TA>), dim3(grid), dim3(256), 0, s, A, ld, sd, dcond, nmat, n, stride);
-    else if (n <= 32)
-      hipLaunchKernelGGL((k_det_value<32, TA>), dim3(grid), dim3(256), 0, s, A, ld, sd, dcond, nmat, n, stride);
+    // 4 wavefronts per workgroup, 64 / NMAX matrices per wavefront
+    if (n <= 8)
+      hipLaunchKernelGGL((k_det_value<8, TA>), dim3((unsigned)((nmat + 31) / 32)), dim3(256), 0, s, A, ld, sd, dcond,
+                         nmat, n, stride);
+    else if (n <= 16)
+      hipLaunchKernelGGL((k_det_value<16, TA>), dim3((unsigned)((nmat + 15) / 16)), dim3(256), 0, s, A, ld, sd, dcond,
+                         nmat, n, stride);
     else
-      hipLaunchKernelGGL((k_det_value<64, TA>), dim3(grid), dim3(256), 0, s, A, ld, sd, dcond, nmat, n, stride);
+      hipLaunchKernelGGL((k_det_value<32, TA>), dim3((unsigned)((nmat + 7) / 8)), dim3(256), 0, s, A, ld, sd, dcond,
+                         nmat, n, stride);
     return;
   }
   if (n <= 16) {
@@ -708,16 +995,11 @@ void launch_det(const TA* A, double* ld, double* sd, double* g, double* lapd, fl
     const size_t shm = (size_t)W * (NM * NM + 64 * NM) * sizeof(double);
     hipLaunchKernelGGL((k_det_lap<NM, W, TA>), dim3((unsigned)((nmat + W - 1) / W)), dim3(64 * W), shm, s, A, ld, sd,
                        g, lapd, dcond, nmat, n, sh.C);
-  } else if (n <= 32) {
+  } else {
     constexpr int NM = 32, W = 2;
     const size_t shm = (size_t)W * (NM * NM + 64 * NM) * sizeof(double);
     hipLaunchKernelGGL((k_det_lap<NM, W, TA>), dim3((unsigned)((nmat + W - 1) / W)), dim3(64 * W), shm, s, A, ld, sd,
                        g, lapd, dcond, nmat, n, sh.C);
-  } else {
-    constexpr int NM = 64, W = 1;
-    const size_t shm = (size_t)W * (NM * NM + 64 * NM) * sizeof(double);
-    hipLaunchKernelGGL((k_det_lap<NM, W, TA>), dim3((unsigned)nmat), dim3(64 * W), shm, s, A, ld, sd, g, lapd, dcond,
-                       nmat, n, sh.C);
   }
 }
 template void launch_det<float>(const float*, double*, double*, double*, double*, float*, const Shape&, hipStream_t);
