@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--cpu-walkers", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--workload", default="n2", choices=["n2", "c4h2-precision"],
+                    help="n2: the headline line; c4h2-precision: the precision-form study (E_loc only)")
+    ap.add_argument("--fp64-threshold", type=float, default=1e6)
     return ap.parse_args()
 
 
@@ -60,8 +63,75 @@ def traffic_from_profiles():
     return None
 
 
+def precision_study(args):
+    """C4H2-like molecule (4 C on a 2.6 bohr rhombus + 2 H), 26 electrons, 16 dets, 4096 Gaussian walkers:
+    E_loc of every walker in each precision form, timed, and compared with the FP64 form on the SAME
+    walkers (the FP64 form matches the FP64 CPU oracle to ~1e-6 Ha; see tests/test_gpu_parity.py)."""
+    import torch
+
+    from deeperwin_amd._native import FLAG_FP64_DET, NativeModel
+    from deeperwin_amd.params import ModelShape, init_params
+    from deeperwin_amd.physics import generate_el_ion_mapping, geometry_c4h2, init_walkers_gaussian
+
+    R, Z = geometry_c4h2(3)
+    n, u, D, B = 26, 13, 16, args.walkers if args.walkers != 2048 else 4096
+    shape = ModelShape(n_el=n, n_up=u, n_ion=6, n_det=D)
+    params = init_params(shape, seed=2)
+    r0 = init_walkers_gaussian(np.random.default_rng(2), B, R, generate_el_ion_mapping(R, Z, n, u))
+    dev = "cuda"
+    r = torch.as_tensor(r0, dtype=torch.float32, device=dev).contiguous()
+    Rd = torch.as_tensor(R, dtype=torch.float32, device=dev)
+    Zd = torch.as_tensor(Z, dtype=torch.float32, device=dev)
+
+    def run(prec, thr, reps):
+        m = NativeModel(shape, prec, fp64_cond_threshold=thr)
+        m.set_params(params)
+        e = torch.empty(B, device=dev)
+        fl = torch.zeros(B, dtype=torch.int32, device=dev)
+        m.local_energy(r, Rd, Zd, e, None, fl)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            m.local_energy(r, Rd, Zd, e, None, fl)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        return e.double().cpu().numpy(), fl.cpu().numpy(), dt
+
+    e64, _, t64 = run("fp64", 0.0, 1)
+    forms = {}
+    for prec, thr in [("fp32", 0.0), ("fp32", args.fp64_threshold), ("bf16x3", 0.0), ("bf16x3", args.fp64_threshold),
+                      ("bf16", args.fp64_threshold)]:
+        e, fl, dt = run(prec, thr, args.steps)
+        de = e - e64
+        key = prec + ("+fp64>%.0e" % thr if thr > 0 else "")
+        forms[key] = {"walker_eloc_per_s": round(B / dt, 1), "ms": round(dt * 1e3, 3),
+                      "dE_mean_mHa": round(float(np.mean(de)) * 1e3, 4),
+                      "median_abs_dE": float(np.median(np.abs(de))), "max_abs_dE": float(np.max(np.abs(de))),
+                      "fp64_recomputed": int(((fl & FLAG_FP64_DET) != 0).sum()) if thr > 0 else 0}
+    base = forms["fp32"]["walker_eloc_per_s"]
+    best = max((k for k in forms if abs(forms[k]["dE_mean_mHa"]) <= 0.1), key=lambda k: forms[k]["walker_eloc_per_s"],
+               default=None)
+    out = {"metric": METRIC, "value": forms[best]["walker_eloc_per_s"] if best else None,
+           "unit": "walker local-energies/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+           "ms_per_step": forms[best]["ms"] if best else None, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": best, "data": "synthetic",
+           "config": {"workload": f"C4H2-like (4 C on a seed-3 rhombus, 2.6 bohr sides, + 2 H), 26 electrons, 16 dets, "
+                                  f"{B} Gaussian walkers, E_loc only", "walkers": B},
+           "reference": {"form": "fp64 (GPU, pinned to the FP64 CPU oracle)", "ms": round(t64 * 1e3, 3),
+                         "energy_mean": float(np.mean(e64))},
+           "forms": forms, "best_form_within_0.1mHa": best,
+           "speedup_vs_plain_fp32": round(forms[best]["walker_eloc_per_s"] / base, 3) if best else None}
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
+    if args.workload == "c4h2-precision":
+        if args.gpus != 1:
+            print("bench.py measures one GPU", file=sys.stderr)
+            sys.exit(2)
+        precision_study(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (walkers shard across GPUs "
               "without any per-step exchange beyond one scalar, see DESIGN.md)", file=sys.stderr)

@@ -8,7 +8,8 @@ FP64 by ``oracle.wavefunction.conditioning``).  Per walker:
   log|psi|^2 : |d| <= 5e-5 max(1, |ref| / 10) + 4 u kappa
   E_loc      : |d| <= (1 + |ref|) (1e-4 + 4 u kappa)
 and over the batch (catches errors that every walker shares):
-  median |d log|psi|^2| <= 5e-4,   median |d E_loc| <= 1e-4 (1 + median |E_loc|)
+  median |d log|psi|^2| <= max(5e-4, tol/20),   median |d E_loc| <= max(1e-4 (1 + median |E_loc|), tol/20)
+(tol/20 = one twentieth of the median per-walker tolerance, for systems conditioned like the n = 34 one)
 MCMC replay: with the same supplied draws >= 98% of walkers follow the oracle's chain exactly (a walker
 can only leave it at a decision within the FP32 noise of the threshold).
 """
@@ -67,7 +68,7 @@ def test_logpsi_matches_oracle(cuda, name, B):
     print(f"{name}: max |d log_psi_sqr| = {err.max():.3e}, median {np.median(err):.2e}, max err/tol "
           f"{np.max(err / _tol_lp(lp_ref, kappa)):.3f}")
     assert np.all(err <= _tol_lp(lp_ref, kappa))
-    assert np.median(err) <= 5e-4
+    assert np.median(err) <= max(5e-4, 0.05 * np.median(_tol_lp(lp_ref, kappa)))
     np.testing.assert_array_equal(ph.cpu().numpy(), ph_ref.numpy().astype(np.float32))
 
 
@@ -87,7 +88,7 @@ def test_local_energy_matches_oracle(cuda, name, B):
     print(f"{name}: E_loc ref {e_ref[:4]} max |dE| = {err.max():.3e}, median {np.median(err):.2e}, "
           f"max err/tol {np.max(err / _tol_e(e_ref, kappa)):.3f}")
     assert np.all(err <= _tol_e(e_ref, kappa))
-    assert np.median(err) <= 1e-4 * (1 + np.median(np.abs(e_ref)))
+    assert np.median(err) <= max(1e-4 * (1 + np.median(np.abs(e_ref))), 0.05 * np.median(_tol_e(e_ref, kappa)))
     lp_ref = _oracle_lp(params, shape, r, R)[1].numpy()
     assert np.all(np.abs(lp.cpu().numpy() - lp_ref) <= _tol_lp(lp_ref, kappa))
     assert int(fl.cpu().numpy().max()) == 0
@@ -129,7 +130,7 @@ def test_mcmc_replay_matches_oracle(cuda):
     assert n == S and abs(s - 0.02) < 1e-7
 
 
-@pytest.mark.parametrize("name,B", [("lih", 16), ("n2", 4)])
+@pytest.mark.parametrize("name,B", [("lih", 16), ("n2", 4), ("big", 2)])
 def test_fp64_form_matches_oracle(cuda, name, B):
     """The whole pass in FP64 (the reference's float64 mode): only the float32 outputs round."""
     shape, params, R, Z, mapping = system(name)
@@ -183,3 +184,26 @@ def test_flagged_walkers_are_recomputed_in_fp64(cuda):
     # the device estimate tracks the FP64 conditioning (1-norm vs 2-norm condition: within a factor ~n)
     k = kap.cpu().numpy()
     assert np.all(k >= kappa_ref / 20) and np.all(k <= kappa_ref * 20)
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "bf16"])
+def test_bf16_forms_track_fp32(cuda, prec):
+    """BF16 MFMA forms (fused Laplacian GEMM included) against the FP64 oracle.  The split form keeps
+    ~16 mantissa bits (u ~ 2^-17), the single form 8 (u ~ 2^-9); tolerances scale accordingly."""
+    shape, params, R, Z, mapping = system("n2")
+    B = 8
+    r = walkers(R, mapping, B, seed=2)
+    m = _native(shape, params, prec)
+    e = torch.empty(B, device="cuda")
+    lp = torch.empty(B, device="cuda")
+    m.local_energy(_dev(r), _dev(R), _dev(Z), e, lp)
+    torch.cuda.synchronize()
+    e_ref = oh.local_energy(params, shape.n_up, shape.n_dn, r.astype(np.float64), R, Z).numpy()
+    lp_ref = _oracle_lp(params, shape, r, R)[1].numpy()
+    kappa = owf.conditioning(params, shape.n_up, shape.n_dn, r, R)
+    u = 2.0 ** -17 if prec == "bf16x3" else 2.0 ** -9
+    de = np.abs(e.cpu().numpy() - e_ref)
+    dl = np.abs(lp.cpu().numpy() - lp_ref)
+    print(f"{prec}: median |dE| {np.median(de):.2e}, max {de.max():.2e}; median |dlp| {np.median(dl):.2e}")
+    assert np.all(dl <= 5e-5 * np.maximum(1, np.abs(lp_ref) / 10) + 4 * u * kappa)
+    assert np.all(de <= (1 + np.abs(e_ref)) * (1e-4 + 4 * u * kappa))
