@@ -46,8 +46,12 @@ def parse():
     ap.add_argument("--cpu-walkers", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--workload", default="n2", choices=["n2", "c4h2-precision"],
-                    help="n2: the headline line; c4h2-precision: the precision-form study (E_loc only)")
+    ap.add_argument("--workload", default="n2", choices=["n2", "c4h2-precision", "serving", "copy"],
+                    help="n2: the headline line; c4h2-precision: the precision-form study (E_loc only); "
+                         "serving: Poisson request stream with continuous batching; copy: a plain device copy "
+                         "of known size (calibrates the HBM byte counters)")
+    ap.add_argument("--requests", type=int, default=24, help="serving: requests in the stream")
+    ap.add_argument("--rate", type=float, default=50.0, help="serving: offered load, requests/s")
     ap.add_argument("--fp64-threshold", type=float, default=1e6)
     return ap.parse_args()
 
@@ -84,7 +88,7 @@ def precision_study(args):
     Zd = torch.as_tensor(Z, dtype=torch.float32, device=dev)
 
     def run(prec, thr, reps):
-        m = NativeModel(shape, prec, fp64_cond_threshold=thr)
+        m = NativeModel(shape, prec, fp64_cond_threshold=thr, chunk=1024)  # bounds the workspace
         m.set_params(params)
         e = torch.empty(B, device=dev)
         fl = torch.zeros(B, dtype=torch.int32, device=dev)
@@ -124,17 +128,87 @@ def precision_study(args):
     print(json.dumps(out))
 
 
+def copy_calibration(args):
+    """A plain device-to-device copy of 256 MiB, launched 5 times: FETCH_SIZE / WRITE_SIZE of this kernel
+    against its known 256 MiB read + 256 MiB write give the scale of the HBM byte counters."""
+    import torch
+
+    x = torch.ones(64 << 20, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": "copy", "bytes_read_per_launch": x.numel() * 4,
+                      "bytes_written_per_launch": y.numel() * 4, "launches": 5}))
+
+
+def serving_bench(args):
+    """Poisson stream of energy-evaluation requests (2-60 electrons, 256-4096 walkers log-uniform, 50 MCMC
+    steps + mean E_loc each) under one shared max-size weight set, served with continuous batching; the
+    same requests are then run one at a time for the isolated latencies, and every request's result is
+    compared between the two runs."""
+    import dataclasses
+
+    import torch
+
+    from deeperwin_amd.serving import Server, SharedWeights, poisson_stream, summarize
+
+    w = SharedWeights(n_el_max=60, n_ion_max=60, n_det=8, seed=4, precision=args.precision, chunk=256)
+    reqs = poisson_stream(args.requests, rate=args.rate, seed=5)
+    warm = poisson_stream(2, rate=1e6, seed=99, n_el_range=(4, 8), walker_range=(256, 256), n_steps=2)
+    Server(w, max_active=2, slice_steps=2).serve(warm, realtime=False)
+    iso_res = {}
+    for q in reqs:
+        q0 = dataclasses.replace(q, arrival=0.0)
+        iso_res[q.rid] = Server(w, max_active=1, slice_steps=q.n_steps).serve([q0], realtime=False)[0]
+        print(f"isolated request {q.rid}: n_el={q.n_el} walkers={q.n_walkers} "
+              f"{iso_res[q.rid].latency * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    isolated = {k: v.latency for k, v in iso_res.items()}
+    busy = sum(isolated.values())
+    capacity = len(reqs) / busy
+    runs = {}
+    for label, rate in (("offered_%g_per_s" % args.rate, args.rate), ("at_0.7_capacity", 0.7 * capacity)):
+        stream = [dataclasses.replace(q, arrival=q.arrival * args.rate / rate) for q in reqs]
+        res = Server(w, max_active=8, slice_steps=10).serve(stream)
+        summ = summarize(res, isolated)
+        walkers = sum(q.n_walkers for q in reqs)
+        summ["rate_per_s"] = round(rate, 3)
+        summ["walker_eloc_per_s"] = round(walkers / summ["makespan_s"], 1)
+        summ["identical_to_isolated"] = all(
+            r.energy_mean == iso_res[r.rid].energy_mean and r.acceptance == iso_res[r.rid].acceptance for r in res)
+        runs[label] = summ
+        print(f"{label}: {json.dumps(summ)}", file=sys.stderr, flush=True)
+    head = runs["offered_%g_per_s" % args.rate]
+    out = {"metric": METRIC, "value": head["walker_eloc_per_s"], "unit": "walker local-energies/s",
+           "n_gpus": 1, "steps": len(reqs), "warmup": len(warm), "ms_per_step": round(1e3 * head["makespan_s"] / len(reqs), 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": args.precision,
+           "data": "synthetic",
+           "config": {"workload": f"serving: {len(reqs)} Poisson requests at {args.rate:g}/s, 2-60 electrons, "
+                                  "256-4096 walkers, 50 MCMC steps + mean E_loc, 8 dets, shared weights",
+                      "max_active": 8, "slice_steps": 10},
+           "isolated": {"p50_s": head["isolated_p50_s"], "p99_s": head["isolated_p99_s"],
+                        "sum_s": round(busy, 4), "capacity_req_per_s": round(capacity, 3)},
+           "runs": runs}
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
+    if args.workload != "n2" and args.gpus != 1:
+        print("bench.py measures one GPU", file=sys.stderr)
+        sys.exit(2)
     if args.workload == "c4h2-precision":
-        if args.gpus != 1:
-            print("bench.py measures one GPU", file=sys.stderr)
-            sys.exit(2)
         precision_study(args)
+        return
+    if args.workload == "serving":
+        serving_bench(args)
+        return
+    if args.workload == "copy":
+        copy_calibration(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (walkers shard across GPUs "
-              "without any per-step exchange beyond one scalar, see DESIGN.md)", file=sys.stderr)
+              "without any per-step exchange beyond one scalar)", file=sys.stderr)
         sys.exit(2)
     import torch
 

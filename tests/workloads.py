@@ -22,6 +22,11 @@ def system(name: str, n_det: int = None, seed: int = 0):
 
         R, Z = random_molecule(np.random.default_rng(34), 34)
         D = n_det or 2
+    elif name == "big60":
+        from deeperwin_amd.physics import random_molecule
+
+        R, Z = random_molecule(np.random.default_rng(60), 60)
+        D = n_det or 2
     elif name == "he":
         R, Z = np.zeros((1, 3)), np.array([2])
         D = n_det or 2
