@@ -46,9 +46,10 @@ def parse():
     ap.add_argument("--cpu-walkers", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--workload", default="n2", choices=["n2", "c4h2-precision", "serving", "copy"],
+    ap.add_argument("--workload", default="n2", choices=["n2", "c4h2-precision", "serving", "batch64", "copy"],
                     help="n2: the headline line; c4h2-precision: the precision-form study (E_loc only); "
-                         "serving: Poisson request stream with continuous batching; copy: a plain device copy "
+                         "serving: Poisson request stream with continuous batching; batch64: 64 geometries "
+                         "under shared weights in one graph launch per unit; copy: a plain device copy "
                          "of known size (calibrates the HBM byte counters)")
     ap.add_argument("--requests", type=int, default=24, help="serving: requests in the stream")
     ap.add_argument("--rate", type=float, default=50.0, help="serving: offered load, requests/s")
@@ -192,6 +193,119 @@ def serving_bench(args):
     print(json.dumps(out))
 
 
+def batch_bench(args):
+    """64 seeded geometries (electron counts uniform in [4, 42], Z in {1, 6, 7, 8}, nearest-neighbour
+    distances 1.8-2.8 bohr), shared random-init weights (256/32 widths, 8 dets, truncated per geometry),
+    512 walkers each; one unit = 20 MCMC steps + E_loc for every geometry, recorded as one HIP graph.
+    Launch shapes come from the autotuner (one representative geometry per electron-count bucket).
+    Compared with the same batch run eagerly geometry after geometry on one stream with the built-in
+    launch-shape heuristics."""
+    import torch
+
+    from deeperwin_amd._native import NativeModel
+    from deeperwin_amd.batching import GeometryBatch
+    from deeperwin_amd.params import ModelShape, init_params, truncate_params
+    from deeperwin_amd.physics import batch_geometries, generate_el_ion_mapping, init_walkers_gaussian
+    from deeperwin_amd.tuning import BUCKETS, bucket_of, tune_buckets
+
+    geoms = batch_geometries(seed=4, n_geom=64, n_min=4, n_max=42)
+    W, S, chunk = 512, args.mcmc_steps, 128
+    n_el_max, n_ion_max, D = 42, max(len(Z) for _, Z in geoms), 8
+    big = init_params(ModelShape(n_el_max, (n_el_max + 1) // 2, n_ion_max, D), 3)
+    reps = {}
+    for R, Z in geoms:  # the largest geometry of each bucket represents it
+        n = int(np.sum(Z))
+        b = bucket_of(n)
+        if b not in reps or n > int(np.sum(reps[b][1])):
+            reps[b] = (R, Z)
+    by_n = {int(np.sum(Z)): (R, Z) for R, Z in reps.values()}
+
+    def make_model(n):
+        R, Z = by_n[n]
+        shape = ModelShape(n, (n + 1) // 2, len(Z), D)
+        m = NativeModel(shape, args.precision, chunk=chunk)
+        m.set_params(truncate_params(big, n_ion_max, n_el_max, shape))
+        r0 = init_walkers_gaussian(np.random.default_rng(0), W, R, generate_el_ion_mapping(R, Z, n, shape.n_up))
+        return (m, torch.as_tensor(r0, dtype=torch.float32, device="cuda").contiguous(),
+                torch.as_tensor(R, dtype=torch.float32, device="cuda"),
+                torch.as_tensor(Z, dtype=torch.float32, device="cuda"))
+
+    t0 = time.perf_counter()
+    table, tune_log = tune_buckets(make_model, sorted(by_n), reps=2)
+    t_tune = time.perf_counter() - t0
+    print(f"tuned {len(by_n)} buckets in {t_tune:.1f} s", file=sys.stderr, flush=True)
+
+    def unit_time(bt, fn, k):
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(7)
+        for _ in range(max(1, args.warmup)):
+            bt.refresh_draws(gen)
+            fn()
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        for _ in range(k):
+            bt.refresh_draws(gen)
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - a) / k
+
+    # plain form: eager, one stream, built-in heuristics
+    plain = GeometryBatch(geoms, n_det=D, walkers=W, n_steps=S, precision=args.precision, seed=3, n_streams=1,
+                          n_el_max=n_el_max, n_ion_max=n_ion_max)
+    for gm in plain.geoms:
+        gm.model.set_chunk(chunk)
+    t_plain = unit_time(plain, plain.run_eager, max(1, args.steps // 2))
+    print(f"plain eager unit {t_plain * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    del plain
+    torch.cuda.empty_cache()
+
+    bt = GeometryBatch(geoms, n_det=D, walkers=W, n_steps=S, precision=args.precision, seed=3, n_streams=8,
+                       table=table, n_el_max=n_el_max, n_ion_max=n_ion_max)
+    for gm in bt.geoms:
+        gm.model.set_chunk(chunk)
+    t_eager = unit_time(bt, bt.run_eager, max(1, args.steps // 2))
+    print(f"tuned multi-stream eager unit {t_eager * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    bt.capture()
+    t_capture = time.perf_counter() - t0
+    t_graph = unit_time(bt, bt.replay, args.steps)
+    print(f"graph unit {t_graph * 1e3:.1f} ms (capture {t_capture:.1f} s)", file=sys.stderr, flush=True)
+
+    # accuracy: FP64 form on the final walkers of 8 geometries (every 8th)
+    acc = []
+    for gm in bt.geoms[::8]:
+        m64 = NativeModel(gm.shape, "fp64", chunk=chunk)
+        m64.set_params(truncate_params(big, n_ion_max, n_el_max, gm.shape))
+        e64 = torch.empty(W, device="cuda")
+        m64.local_energy(gm.r, gm.R, gm.Z, e64)
+        torch.cuda.synchronize()
+        a, b = gm.e.double().cpu().numpy(), e64.double().cpu().numpy()
+        ok = np.isfinite(a) & np.isfinite(b)
+        acc.append({"n_el": gm.shape.n_el, "dE_mean_mHa": round(float(np.mean(a[ok]) - np.mean(b[ok])) * 1e3, 4),
+                    "median_abs_dE": float(np.median(np.abs(a[ok] - b[ok])))})
+    walkers = W * len(geoms)
+    flops = bt.flops_per_unit()
+    peak = PEAK_TFLOPS.get(args.precision, 157.3)
+    achieved = flops / t_graph / 1e12
+    out = {"metric": METRIC, "value": round(walkers / t_graph, 1), "unit": "walker local-energies/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_graph * 1e3, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": args.precision,
+           "data": "synthetic",
+           "config": {"workload": "64 seeded geometries, 4-42 electrons, shared weights 256/32, 8 dets, 512 walkers "
+                                  f"each; one unit = {S} MCMC steps + E_loc per geometry in one graph launch",
+                      "geometries": len(geoms), "walkers_per_geometry": W, "mcmc_steps": S,
+                      "n_el": sorted(int(np.sum(Z)) for _, Z in geoms), "eloc_chunk": chunk},
+           "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
+                        "frac": round(achieved / peak, 4), "traffic": None,
+                        "scope": "whole unit: algorithmic FLOPs (n_steps forwards + one Laplacian pass per walker) "
+                                 "/ wall time of a graph replay"},
+           "plain_eager_ms": round(t_plain * 1e3, 3), "tuned_eager_ms": round(t_eager * 1e3, 3),
+           "graph_capture_s": round(t_capture, 2), "speedup_vs_plain": round(t_plain / t_graph, 3),
+           "tuning": {"seconds": round(t_tune, 1), "table": json.loads(table.to_json()), "log": tune_log},
+           "accuracy_vs_fp64_form": acc}
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.workload != "n2" and args.gpus != 1:
@@ -202,6 +316,9 @@ def main():
         return
     if args.workload == "serving":
         serving_bench(args)
+        return
+    if args.workload == "batch64":
+        batch_bench(args)
         return
     if args.workload == "copy":
         copy_calibration(args)
@@ -274,7 +391,7 @@ def main():
     peak = PEAK_TFLOPS[args.precision]
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(achieved / peak, 4), "traffic": traffic_from_profiles(),
-                "kernel": "1-el embedding GEMM, Laplacian mode (k_gemm_f32<2>)",
+                "kernel": "1-el embedding GEMM, Laplacian mode (k_gemm_lap)",
                 "flops_per_launch": flops, "avg_launch_ms": round(avg_s * 1e3, 4), "launches": gemm_n}
 
     out = {

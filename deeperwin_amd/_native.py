@@ -32,6 +32,17 @@ class ModelDesc(ctypes.Structure):
                 ("n_iter", ctypes.c_int32), ("precision", ctypes.c_int32), ("fp64_cond_threshold", ctypes.c_float)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("fwd_main_tile", ctypes.c_int32), ("fwd_orb_tile", ctypes.c_int32),
+                ("fwd_mu_split", ctypes.c_int32), ("lap_orb_tile", ctypes.c_int32),
+                ("lap_mu_tile", ctypes.c_int32), ("lap_electrons", ctypes.c_int32)]
+
+
+TUNING_FIELDS = [f for f, _ in Tuning._fields_]
+TUNING_DEFAULT = {"fwd_main_tile": -1, "fwd_orb_tile": -1, "fwd_mu_split": 0, "lap_orb_tile": -1,
+                  "lap_mu_tile": -1, "lap_electrons": 0}
+
+
 SIGNATURES = {
     "dw_model_create": ([ctypes.POINTER(ModelDesc), ctypes.POINTER(_vp)], ctypes.c_int),
     "dw_model_destroy": ([_vp], ctypes.c_int),
@@ -44,6 +55,8 @@ SIGNATURES = {
                        _vp, _vp], ctypes.c_int),
     "dw_workspace_size": ([_vp, _i32, _i32], ctypes.c_size_t),
     "dw_model_set_chunk": ([_vp, _i32], ctypes.c_int),
+    "dw_model_set_tuning": ([_vp, ctypes.POINTER(Tuning)], ctypes.c_int),
+    "dw_model_get_tuning": ([_vp, ctypes.POINTER(Tuning)], ctypes.c_int),
     "dw_model_conditioning": ([_vp, _vp, _i32, _vp], ctypes.c_int),
     "dw_model_set_timing": ([_vp, _i32], ctypes.c_int),
     "dw_model_timer_query": ([_vp, _i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)],
@@ -131,6 +144,21 @@ class NativeModel:
         if not self.lib.dw_model_ready(self.handle):
             raise DwError("not every parameter leaf of the wavefunction was provided")
         self._param_token = token
+
+    def set_tuning(self, **choices) -> None:
+        """Launch-shape choices (fields of dw_tuning); unnamed fields keep their current value."""
+        cur = self.get_tuning()
+        unknown = set(choices) - set(TUNING_FIELDS)
+        if unknown:
+            raise KeyError(f"unknown tuning fields {sorted(unknown)}")
+        cur.update(choices)
+        t = Tuning(*[int(cur[f]) for f in TUNING_FIELDS])
+        _check(self.lib.dw_model_set_tuning(self.handle, ctypes.byref(t)))
+
+    def get_tuning(self) -> Dict[str, int]:
+        t = Tuning()
+        _check(self.lib.dw_model_get_tuning(self.handle, ctypes.byref(t)))
+        return {f: int(getattr(t, f)) for f in TUNING_FIELDS}
 
     def set_chunk(self, walkers: int) -> None:
         _check(self.lib.dw_model_set_chunk(self.handle, int(walkers)))

@@ -136,6 +136,7 @@ struct dw_model {
   std::vector<std::pair<std::string, std::vector<int64_t>>> expected;
   bool dirty = true;
   int chunk = 0;
+  dw_tuning tune{-1, -1, 0, -1, -1, 0};
   GemmPrec prec = GEMM_F32;
   Weights wf, wd;    // float and double layouts
   Workspace sf, sd;  // float and double activations
@@ -328,7 +329,7 @@ size_t ws_bytes(const dw_model* m, long B, int C, size_t esz) {
     tot += (size_t)rows * m->Kx[l] * esz;
     kmmax = std::max(kmmax, m->Km[l]);
   }
-  tot += (size_t)B * Cp * kmmax * esz + (size_t)B * Cp * m->W1 * esz;
+  tot += (size_t)B * Cp * kmmax * esz + (size_t)B * Cp * m->W1 * esz * (C == 1 ? 4 : 1);
   if (esz == 8 && C > 1) tot += (size_t)rows * m->W1 * esz;  // unfused pre-activations (FP64 form)
   tot += (size_t)rows * m->W1 * esz + (size_t)rows * m->NO * esz;
   tot += (size_t)B * m->D * C * m->n * m->n * esz;
@@ -346,7 +347,7 @@ dw_status ensure_ws(dw_model* m, Workspace& w, long B, int C, size_t esz) {
   int kmmax = 0;
   for (int l = 0; l < m->L; ++l) kmmax = std::max(kmmax, m->Km[l]);
   HIP_TRY(w.MU.ensure((size_t)B * Cp * kmmax * esz));
-  HIP_TRY(w.V.ensure((size_t)B * Cp * m->W1 * esz));
+  HIP_TRY(w.V.ensure((size_t)B * Cp * m->W1 * esz * (C == 1 ? 4 : 1)));  // value pass: up to 4 K-split parts
   if (esz == 8 && C > 1) HIP_TRY(w.Z.ensure((size_t)rows * m->W1 * esz));
   HIP_TRY(w.H.ensure((size_t)rows * m->W1 * esz));
   HIP_TRY(w.O.ensure((size_t)rows * m->NO * esz));
@@ -419,12 +420,27 @@ dw_status run_pass(dw_model* m, const float* r, long B, const float* R, const fl
   for (int l = 0; l < m->L; ++l) X[l] = ws.X[l].as<T>();
   launch_features0<T>(r, R, X[0], m->Kx[0], sh, s);
   launch_twoel<T>(r, X.data(), m->Kx.data(), m->Kh.data(), W.W2e.as<T>(), W.b2e.as<T>(), sh, s);
+  const long vstride = B * Cp * (long)m->W1;
   for (int l = 0; l < m->L; ++l) {
     launch_spin_mean<T>(X[l], m->Kx[l], m->Kh[l], ws.MU.as<T>(), m->Km[l], sh, s);
+    // value pass: split K of the per-walker spin-mean GEMM so that it fills the chip (its M is only B)
+    int vs = 1;
+    if (!DBL && C == 1 && m->prec == GEMM_F32) {
+      if (m->tune.fwd_mu_split > 0) {
+        vs = m->tune.fwd_mu_split;
+      } else {
+        const long tiles64 = ((B * Cp + 63) / 64) * ((m->W1 + 63) / 64);
+        vs = tiles64 >= 512 ? 1 : (tiles64 >= 256 ? 2 : 4);
+      }
+      while (vs > 1 && m->Km[l] % (16 * vs) != 0) vs >>= 1;
+    }
     GemmEpiT<T> ev;
     ev.bias = W.bias[l].as<T>();
     ev.C = Cp;
     ev.nC = m->n * Cp;
+    ev.ksplit = vs;
+    ev.cstride = vstride;
+    ev.tile = C == 1 ? -1 : m->tune.lap_mu_tile;
     gemm_t(m, ws.MU.as<T>(), m->Km[l], W.Wmu[l].as<T>(), W.Wmu_hi[l], W.Wmu_lo[l], m->Km[l], ws.V.as<T>(), m->W1,
            B * Cp, m->W1, m->Km[l], ev, s);
     const bool last = (l == m->L - 1);
@@ -438,7 +454,10 @@ dw_status run_pass(dw_model* m, const float* r, long B, const float* R, const fl
     em.n = m->n;
     em.u = m->u;
     em.B = (int)B;
+    em.vsplit = vs;
+    em.vstride = vstride;
     if (C == 1) {
+      em.tile = m->tune.fwd_main_tile;
       em.act_tanh = 1;
       em.permute = last ? 1 : 0;
       gemm_t(m, X[l], m->Kx[l], W.Wmain[l].as<T>(), W.Wmain_hi[l], W.Wmain_lo[l], m->Kx[l], outp, ldo, rows, m->W1,
@@ -451,13 +470,15 @@ dw_status run_pass(dw_model* m, const float* r, long B, const float* R, const fl
         launch_tanh_rule<T>(ws.Z.as<T>(), outp, ldo, last ? 1 : 0, sh, s);
       } else {
         gemm_lap(m->prec, X[l], m->Kx[l], W.Wmain[l].as<float>(), W.Wmain_hi[l].as<uint16_t>(),
-                 W.Wmain_lo[l].as<uint16_t>(), m->Kx[l], ws.V.as<float>(), m->W1, outp, ldo, last ? 1 : 0, sh, s);
+                 W.Wmain_lo[l].as<uint16_t>(), m->Kx[l], ws.V.as<float>(), m->W1, outp, ldo, last ? 1 : 0, sh,
+                 m->tune.lap_electrons, s);
       }
     }
   }
   // orbital GEMMs on the spin-major rows
   const long up_rows = B * m->u * Cp, dn_rows = B * m->dn * Cp;
   GemmEpiT<T> e0;
+  e0.tile = C == 1 ? m->tune.fwd_orb_tile : m->tune.lap_orb_tile;
   gemm_t(m, ws.H.as<T>(), m->W1, W.Wo[0].as<T>(), W.Wo_hi[0], W.Wo_lo[0], m->W1, ws.O.as<T>(), m->NO, up_rows, m->NO,
          m->W1, e0, s);
   gemm_t(m, ws.H.as<T>() + up_rows * m->W1, m->W1, W.Wo[1].as<T>(), W.Wo_hi[1], W.Wo_lo[1], m->W1,
@@ -606,6 +627,25 @@ int32_t dw_model_ready(const dw_model* m) {
     if (it == m->leaves.end() || !it->second.set) return 0;
   }
   return 1;
+}
+
+dw_status dw_model_set_tuning(dw_model* m, const dw_tuning* t) {
+  if (!m || !t) return fail(DW_ERR_INVALID, "null argument");
+  for (int32_t v : {t->fwd_main_tile, t->fwd_orb_tile, t->lap_orb_tile, t->lap_mu_tile})
+    if (v < -1 || v > 2) return fail(DW_ERR_INVALID, "tile must be -1, 0, 1 or 2");
+  if (!(t->fwd_mu_split == 0 || t->fwd_mu_split == 1 || t->fwd_mu_split == 2 || t->fwd_mu_split == 4))
+    return fail(DW_ERR_INVALID, "fwd_mu_split must be 0 (auto), 1, 2 or 4");
+  const int Cp = rows_per_electron(3 * m->n + 2);
+  if (t->lap_electrons < 0 || t->lap_electrons * Cp > 192)
+    return fail(DW_ERR_INVALID, "lap_electrons * pad16(3 n + 2) must be <= 192");
+  m->tune = *t;
+  return DW_OK;
+}
+
+dw_status dw_model_get_tuning(const dw_model* m, dw_tuning* out) {
+  if (!m || !out) return fail(DW_ERR_INVALID, "null argument");
+  *out = m->tune;
+  return DW_OK;
 }
 
 dw_status dw_model_set_chunk(dw_model* m, int32_t max_walkers_per_pass) {

@@ -17,31 +17,35 @@ namespace dwmi {
 
 enum { EPI_BIAS = 1, EPI_V = 2, EPI_TANH = 4, EPI_PERM = 8 };
 
-template <typename T>
-__device__ __forceinline__ long out_row(long row, const GemmEpiT<T>& ep) {
-  // row = (b * n + i) * C + ch  ->  spin-major order
-  const long ch = row % ep.C;
-  const long bi = row / ep.C;
-  const long i = bi % ep.n, b = bi / ep.n;
-  const long dn = ep.n - ep.u;
-  if (i < ep.u) return (b * ep.u + i) * ep.C + ch;
-  return (long)ep.B * ep.u * ep.C + (b * dn + (i - ep.u)) * ep.C + ch;
-}
-
 __device__ __forceinline__ float act_tanh(float v) { return tanhf(v); }
 __device__ __forceinline__ double act_tanh(double v) { return tanh(v); }
 
+// Epilogue of one element at row r = (b * n + i) * C + ch, column c.  Index math in 32 bits (M < 2^31).
 template <int FLAGS, typename T>
-__device__ __forceinline__ void epi_store(T v, long r, int c, T* __restrict__ Cm, int ldc, const GemmEpiT<T>& ep) {
-  if ((FLAGS & EPI_BIAS) && ep.bias) {
-    if ((r % ep.C) == 0) v += ep.bias[c];
+__device__ __forceinline__ void epi_store(T v, int r, int c, T* __restrict__ Cm, int ldc, const GemmEpiT<T>& ep) {
+  int bi = r, ch = 0;
+  if ((FLAGS & (EPI_BIAS | EPI_V | EPI_PERM)) && ep.C > 1) {
+    bi = r / ep.C;
+    ch = r - bi * ep.C;
   }
+  if ((FLAGS & EPI_BIAS) && ep.bias) {
+    if (ch == 0) v += ep.bias[c];
+  }
+  int b = 0;
+  if ((FLAGS & (EPI_V | EPI_PERM)) && (ep.V || ep.permute)) b = bi / ep.n;
   if ((FLAGS & EPI_V) && ep.V) {
-    const long vr = (r / ep.nC) * ep.C + (r % ep.C);
-    v += ep.V[vr * ep.ldv + c];
+    const T* Vp = ep.V + ((long)b * ep.C + ch) * ep.ldv + c;
+    T vs = Vp[0];
+    for (int q = 1; q < ep.vsplit; ++q) vs += Vp[q * ep.vstride];
+    v += vs;
   }
   if ((FLAGS & EPI_TANH) && ep.act_tanh) v = act_tanh(v);
-  const long orow = ((FLAGS & EPI_PERM) && ep.permute) ? out_row(r, ep) : r;
+  long orow = r;
+  if ((FLAGS & EPI_PERM) && ep.permute) {
+    const int i = bi - b * ep.n, dn = ep.n - ep.u;
+    orow = (i < ep.u) ? ((long)b * ep.u + i) * ep.C + ch
+                      : (long)ep.B * ep.u * ep.C + ((long)b * dn + (i - ep.u)) * ep.C + ch;
+  }
   Cm[orow * ldc + c] = v;
 }
 
@@ -65,8 +69,16 @@ __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ A, i
   constexpr int AIT = (AV + 255) / 256, BIT = (BV + 255) / 256;
   __shared__ float As[BK][BM + PAD];
   __shared__ float Bs[BK][BN + PAD];
-  const int tile = tile_index(blockIdx.x, tilesTotal);
+  // split-K: workgroup range [split * gridTiles, (split + 1) * gridTiles) computes K slice `split`
+  const int gridTiles = (tilesTotal + 7) / 8 * 8;
+  const int split = blockIdx.x / gridTiles;
+  const int tile = tile_index(blockIdx.x - split * gridTiles, tilesTotal);
   if (tile >= tilesTotal) return;
+  const int Ks = K / ep.ksplit;
+  A += (long)split * Ks;
+  Bt += (long)split * Ks;
+  Cm += split * ep.cstride;
+  K = Ks;
   const long row0 = (long)(tile / tilesN) * BM;
   const int col0 = (tile % tilesN) * BN;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -153,9 +165,14 @@ __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ A, i
     for (int ni = 0; ni < WN; ++ni)
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
-        const long r = row0 + wm * 32 * WM + mi * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+        const int r = (int)row0 + wm * 32 * WM + mi * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
         const int c = col0 + wn * 32 * WN + ni * 32 + (lane & 31);
-        if (r < M && c < N) epi_store<FLAGS, float>(acc[mi][ni][j], r, c, Cm, ldc, ep);
+        if (r < M && c < N) {
+          if (split == 0)
+            epi_store<FLAGS, float>(acc[mi][ni][j], r, c, Cm, ldc, ep);
+          else
+            Cm[(long)r * ldc + c] = acc[mi][ni][j];
+        }
       }
 }
 
@@ -172,8 +189,17 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const float* __restrict__ A, 
   __shared__ uint16_t Al[X3 ? BM : 1][LDK];
   __shared__ uint16_t Bh[BN][LDK];
   __shared__ uint16_t Bl[X3 ? BN : 1][LDK];
-  const int tile = tile_index(blockIdx.x, tilesTotal);
+  // split-K: workgroup range [split * gridTiles, (split + 1) * gridTiles) computes K slice `split`
+  const int gridTiles = (tilesTotal + 7) / 8 * 8;
+  const int split = blockIdx.x / gridTiles;
+  const int tile = tile_index(blockIdx.x - split * gridTiles, tilesTotal);
   if (tile >= tilesTotal) return;
+  const int Ks = K / ep.ksplit;
+  A += (long)split * Ks;
+  Bhi += (long)split * Ks;  // (the BF16 launcher never splits K: split == 0)
+  if (Blo) Blo += (long)split * Ks;
+  Cm += split * ep.cstride;
+  K = Ks;
   const long row0 = (long)(tile / tilesN) * BM;
   const int col0 = (tile % tilesN) * BN;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -270,7 +296,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const float* __restrict__ A, 
     for (int ni = 0; ni < 2; ++ni)
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
-        const long r = row0 + wm * 64 + mi * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+        const int r = (int)row0 + wm * 64 + mi * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
         const int c = col0 + wn * 64 + ni * 32 + (lane & 31);
         if (r < M && c < N) epi_store<FLAGS, float>(acc[mi][ni][j], r, c, Cm, ldc, ep);
       }
@@ -330,7 +356,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const double* __restrict__ A, 
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const long r = row0 + ty + 16 * i;
+      const int r = (int)row0 + ty + 16 * i;
       const int c = col0 + tx + 16 * j;
       if (r < M && c < N) epi_store<FLAGS, double>(acc[i][j], r, c, Cm, ldc, ep);
     }
@@ -352,7 +378,7 @@ static void launch_f32(const float* A, int lda, const float* Bt, int ldb, float*
                        const GemmEpi& ep, hipStream_t s) {
   const int tilesM = (M + 64 * WM - 1) / (64 * WM), tilesN = (N + 64 * WN - 1) / (64 * WN);
   const int total = tilesM * tilesN;
-  const int grid = (total + 7) / 8 * 8;
+  const int grid = (total + 7) / 8 * 8 * ep.ksplit;
   hipLaunchKernelGGL((k_gemm_f32<FLAGS, WM, WN>), dim3(grid), dim3(256), 0, s, A, lda, Bt, ldb, C, ldc, M, N, K,
                      tilesN, total, ep);
 }
@@ -372,7 +398,7 @@ static void gemm_dispatch(GemmPrec prec, const float* A, int lda, const float* B
                           hipStream_t s) {
   if (prec == GEMM_F32) {
     // Bt holds pad128(N) rows, so every tile shape reads inside it
-    switch (pick_tile(M, N)) {
+    switch (ep.tile >= 0 ? ep.tile : pick_tile((long)M * ep.ksplit, N)) {
       case 2: launch_f32<FLAGS, 2, 2>(A, lda, Bt, ldb, C, ldc, M, N, K, ep, s); break;
       case 1: launch_f32<FLAGS, 1, 2>(A, lda, Bt, ldb, C, ldc, M, N, K, ep, s); break;
       default: launch_f32<FLAGS, 1, 1>(A, lda, Bt, ldb, C, ldc, M, N, K, ep, s); break;
@@ -435,7 +461,9 @@ __global__ __launch_bounds__(256) void k_gemm_lap(const float* __restrict__ X, i
   constexpr int BN = 128;
   constexpr int BK = (PREC == 0) ? 16 : 32;
   constexpr int LDK = BK + 8;  // bf16 row pitch (elements)
-  constexpr int EP = 64 + 4;   // epilogue pitch (floats): 64 columns per pass
+  // epilogue window: all 128 columns at once when the tile fits in 60 KB of LDS, else two passes of 64
+  constexpr int NC = ((size_t)RB * 16 * (128 + 4) * 4 <= 60 * 1024) ? 128 : 64;
+  constexpr int EP = NC + 4;   // epilogue pitch (floats)
   constexpr size_t LOAD_BYTES = (PREC == 0)
                                     ? (size_t)BK * (ROWS + 4) * 4 + (size_t)BK * (BN + 4) * 4
                                     : (size_t)(ROWS + BN) * LDK * 2 * (PREC == 1 ? 2 : 1);
@@ -607,45 +635,84 @@ __global__ __launch_bounds__(256) void k_gemm_lap(const float* __restrict__ X, i
       }
     }
   }
-  // ---- epilogue: tile to LDS (two passes of 64 columns), one thread per (electron, column) applies the rule
+  // ---- epilogue: tile to LDS, then (electron, column) pairs with lanes along columns, so every store of
+  // a wave is one contiguous 256-byte row segment.  When a pass has fewer than 256 pairs, G threads share
+  // a pair and split its tangent channels into contiguous chunks; their partial sum_k z_k^2 meet in LDS
+  // in a fixed order (deterministic).  V and z of 8 channels are loaded before any of them is used.
+  __shared__ float red[256];
   float(*Es)[EP] = (float(*)[EP])smem;
-  const int cc = tid & 63;
-  const int col = col0 + (cc >> 4) * 32 + (cc & 15);  // + 16 * q below
   const int dn = n - sh.u;
+  const int P = E * NC;
+  const int G = P >= 256 ? 1 : 256 / P;
+  const int nt = C - 2;
+  const int chunk = (nt + G - 1) / G;
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
+  for (int pass = 0; pass < 128 / NC; ++pass) {
     __syncthreads();
+    const int wc0 = wv * 32 - pass * NC;  // this wave's first column inside the pass window
+    if (wc0 >= 0 && wc0 < NC) {
 #pragma unroll
-    for (int rb2 = 0; rb2 < RB; ++rb2)
+      for (int rb2 = 0; rb2 < RB; ++rb2)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) Es[rb2 * 16 + 4 * (lane >> 4) + j][wv * 16 + (lane & 15)] = acc[rb2][q][j];
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) Es[rb2 * 16 + 4 * (lane >> 4) + j][wc0 + q * 16 + (lane & 15)] = acc[rb2][q][j];
+    }
     __syncthreads();
-    const int gc = col + 16 * q;
-    for (int e = tid >> 6; e < E; e += 4) {
+    for (int t0 = 0; t0 < P * G; t0 += 256) {
+      const int t = t0 + tid;
+      const int grp = t / P, p = t - (t / P) * P;
+      const int e = p / NC, c = p - (p / NC) * NC;
       const long g = g0 + e;
-      if (g >= n_el_total) break;
-      const long b = g / n;
-      const int i = (int)(g % n);
-      const float* Vb = V + (b * Cp) * (long)ldv + gc;
-      long obase;
-      if (permute)
-        obase = (i < sh.u) ? (b * sh.u + i) * Cp : (long)sh.B * sh.u * Cp + (b * dn + (i - sh.u)) * Cp;
-      else
-        obase = g * Cp;
-      const float* Ez = &Es[e * Cp][cc];
-      const float z0 = Ez[0] + Vb[0];
-      const float th = tanhf(z0);
-      const float d1 = 1.f - th * th, d2 = -2.f * th * d1;
-      out[obase * ldo + gc] = th;
-      float S = 0.f;
-      for (int ch = 1; ch < C - 1; ++ch) {
-        const float zc = Ez[ch * EP] + Vb[(long)ch * ldv];
-        S += zc * zc;
-        out[(obase + ch) * ldo + gc] = d1 * zc;
+      const bool valid = (t < P * G) && (g < n_el_total);
+      const int gc = col0 + pass * NC + c;
+      long obase = 0;
+      const float* Vb = V;
+      const float* Ez = &Es[0][0];
+      float d1 = 0.f, d2 = 0.f, S = 0.f;
+      if (valid) {
+        const long b = g / n;
+        const int i = (int)(g - b * n);
+        Vb = V + (b * Cp) * (long)ldv + gc;
+        if (permute)
+          obase = (i < sh.u) ? (b * sh.u + i) * Cp : (long)sh.B * sh.u * Cp + (b * dn + (i - sh.u)) * Cp;
+        else
+          obase = g * Cp;
+        Ez = &Es[e * Cp][c];
+        const float th = tanhf(Ez[0] + Vb[0]);
+        d1 = 1.f - th * th;
+        d2 = -2.f * th * d1;
+        if (grp == 0) out[obase * ldo + gc] = th;
+        const int c_lo = 1 + grp * chunk;
+        const int c_hi = min(C - 1, c_lo + chunk);
+        int ch = c_lo;
+        for (; ch + 8 <= c_hi; ch += 8) {
+          float z[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) z[u] = Ez[(ch + u) * EP] + Vb[(long)(ch + u) * ldv];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            S += z[u] * z[u];
+            out[(obase + ch + u) * ldo + gc] = d1 * z[u];
+          }
+        }
+        for (; ch < c_hi; ++ch) {
+          const float zc = Ez[ch * EP] + Vb[(long)ch * ldv];
+          S += zc * zc;
+          out[(obase + ch) * ldo + gc] = d1 * zc;
+        }
       }
-      const float zl = Ez[(C - 1) * EP] + Vb[(long)(C - 1) * ldv];
-      out[(obase + C - 1) * ldo + gc] = d1 * zl + d2 * S;
-      for (int ch = C; ch < Cp; ++ch) out[(obase + ch) * ldo + gc] = 0.f;
+      if (G > 1) {
+        red[tid] = S;
+        __syncthreads();
+        S = 0.f;
+        for (int q = 0; q < G; ++q) S += red[q * P + p];
+      }
+      if (valid && grp == 0) {
+        const float zl = Ez[(C - 1) * EP] + Vb[(long)(C - 1) * ldv];
+        out[(obase + C - 1) * ldo + gc] = d1 * zl + d2 * S;
+        for (int ch = C; ch < Cp; ++ch) out[(obase + ch) * ldo + gc] = 0.f;
+      }
     }
   }
 }
@@ -686,8 +753,9 @@ int lap_electrons_per_tile(int Cp) {
 }
 
 void gemm_lap(GemmPrec prec, const float* X, int ldx, const float* Bt, const uint16_t* Bhi, const uint16_t* Blo, int K,
-              const float* V, int ldv, float* out, int ldo, int permute, const Shape& sh, hipStream_t s) {
-  const int E = lap_electrons_per_tile(sh.Cp);
+              const float* V, int ldv, float* out, int ldo, int permute, const Shape& sh, int E_override,
+              hipStream_t s) {
+  const int E = E_override > 0 ? E_override : lap_electrons_per_tile(sh.Cp);
   const int RB = E * sh.Cp / 16;
   if (prec == GEMM_F32)
     gemm_lap_rb<0>(RB, X, ldx, Bt, Bhi, Blo, K, V, ldv, out, ldo, permute, E, sh, s);

@@ -16,6 +16,11 @@ struct GemmEpiT {
   int act_tanh = 0;             // y = tanh(.)  (value-only forward)
   int permute = 0;              // output rows go to spin-major order (up block, then down block)
   int n = 0, u = 0, B = 0;      // shape for `permute`
+  int vsplit = 1;               // V is the sum of vsplit partial buffers V + s * vstride
+  long vstride = 0;
+  int tile = -1;                // FP32 tile: -1 = heuristic, 0 = 64x64, 1 = 64x128, 2 = 128x128
+  int ksplit = 1;               // split K over ksplit partial outputs C + s * cstride (bias on s = 0 only)
+  long cstride = 0;
 };
 using GemmEpi = GemmEpiT<float>;
 using GemmEpiD = GemmEpiT<double>;
@@ -39,8 +44,11 @@ struct Shape {
 };
 
 // Fused 1-el GEMM + tanh forward-Laplacian rule of the Laplacian pass (rows: Cp per electron).
+// E_override > 0 sets the electrons per workgroup (E * Cp / 16 <= 12); 0 picks E = max(1, 96 / Cp).
 void gemm_lap(GemmPrec prec, const float* X, int ldx, const float* Bt, const uint16_t* Bhi, const uint16_t* Blo, int K,
-              const float* V, int ldv, float* out, int ldo, int permute, const Shape& sh, hipStream_t s);
+              const float* V, int ldv, float* out, int ldo, int permute, const Shape& sh, int E_override,
+              hipStream_t s);
+int lap_electrons_per_tile(int Cp);
 
 template <typename T>
 void launch_features0(const float* r, const float* R, T* X0, int Kx0, const Shape& sh, hipStream_t s);

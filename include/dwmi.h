@@ -117,6 +117,20 @@ size_t dw_workspace_size(const dw_model* m, int32_t n_walkers, int32_t mode);
 /* Cap on walkers processed per internal pass of dw_local_energy (0 = no cap).  Bounds the workspace. */
 dw_status dw_model_set_chunk(dw_model* m, int32_t max_walkers_per_pass);
 
+/* Launch-shape choices of one model (no counterpart in the reference: XLA picks its own tiles).  Set by
+ * the autotuner in deeperwin_amd/tuning.py per electron-count bucket; -1 / 0 keep the built-in heuristic.
+ * FP32 GEMM tiles: 0 = 64x64, 1 = 64x128, 2 = 128x128 per 256-thread workgroup. */
+typedef struct dw_tuning {
+  int32_t fwd_main_tile;  /* 1-el embedding GEMMs of the value pass                                  */
+  int32_t fwd_orb_tile;   /* orbital GEMMs of the value pass                                         */
+  int32_t fwd_mu_split;   /* K-split (1, 2 or 4) of the spin-mean GEMMs of the value pass (FP32 form) */
+  int32_t lap_orb_tile;   /* orbital GEMMs of the Laplacian pass                                     */
+  int32_t lap_mu_tile;    /* spin-mean GEMMs of the Laplacian pass                                   */
+  int32_t lap_electrons;  /* electrons per workgroup of the fused Laplacian GEMM (E * Cp <= 192)      */
+} dw_tuning;
+dw_status dw_model_set_tuning(dw_model* m, const dw_tuning* t);
+dw_status dw_model_get_tuning(const dw_model* m, dw_tuning* out);
+
 /* ---- Measurement hooks (no counterpart in the reference; used by the benchmark harness) ----
  * When timing is enabled the library brackets selected launches with HIP events recorded on the
  * launch stream; dw_model_timer_query synchronises those events and returns the summed duration and the
