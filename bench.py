@@ -54,6 +54,7 @@ def parse():
     ap.add_argument("--requests", type=int, default=24, help="serving: requests in the stream")
     ap.add_argument("--rate", type=float, default=50.0, help="serving: offered load, requests/s")
     ap.add_argument("--fp64-threshold", type=float, default=1e6)
+    ap.add_argument("--no-tune", action="store_true", help="keep the built-in launch-shape heuristics")
     return ap.parse_args()
 
 
@@ -359,6 +360,12 @@ def main():
     gen = torch.Generator(device=dev)
     gen.manual_seed(1)
 
+    tuning = None
+    if not args.no_tune:  # launch shapes from the autotuner, on the walkers of this run
+        from deeperwin_amd.tuning import autotune_model
+
+        tuning, _ = autotune_model(model, r, Rd, Zd, reps=3)
+
     def step():
         normals = torch.randn((S, B, n, 3), generator=gen, device=dev)
         uniforms = torch.rand((S, B), generator=gen, device=dev)
@@ -402,7 +409,7 @@ def main():
                                f"16 full dets, random init seed 1; {B} walkers; {S} MCMC steps (stepsize 0.02) + "
                                f"E_loc with the full Laplacian over {3 * n} directions per step",
                    "walkers": B, "mcmc_steps": S, "n_el": n, "n_det": D, "precision": args.precision,
-                   "parallelism": "single GPU (walker-parallel)"},
+                   "parallelism": "single GPU (walker-parallel)", "launch_shapes": tuning or "heuristic"},
         "roofline": roofline,
         "breakdown": {"forward_pass_ms": round(fwd_ms / max(fwd_n, 1), 4),
                       "local_energy_pass_ms": round(lap_ms / max(lap_n, 1), 4),

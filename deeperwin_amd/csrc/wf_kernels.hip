@@ -248,9 +248,294 @@ __global__ __launch_bounds__(256) void k_twoel(const float* __restrict__ r, TwoE
   }
 }
 
+// ------------------------------------------------------------------------------------------------
+// FP32 two-electron stream on MFMA.  One wavefront per (walker b, group of EPW electrons): the 32 rows
+// of a v_mfma_f32_32x32x2_f32 tile are (electron e of the group, partner j) with P = 32 / EPW partners
+// per electron (EPW = 4, 2, 1 for n <= 8, 16, > 16; n > 32 uses two row tiles), the 32 features are its
+// columns; each layer's weights are the B operand, held in registers.  The value, the 3 tangents along
+// r_j - r_i and the Laplacian are 5 channels sharing those weights (no bias on derivative channels).
+// A layer's output (C layout: lane = feature, 16 rows per lane) is transposed through wave-private LDS
+// into the A layout of the next layer.  Spin sums over partners accumulate per lane over its rows and
+// meet across lanes in a fixed order.  Same quantities as k_twoel: the self pair is included in the
+// means, padded rows are excluded.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ int mfma_row(int q, int h) { return 8 * (q >> 2) + 4 * h + (q & 3); }
+
+template <bool LAP, int L, int EPW>
+__global__ __launch_bounds__(64) void k_twoel_mfma(const float* __restrict__ r, TwoElOut<float> out,
+                                                   const float* __restrict__ W2e, const float* __restrict__ b2e,
+                                                   Shape sh) {
+  constexpr int NCH = LAP ? 5 : 1;
+  constexpr int NL2 = L - 1;  // 2-el layers
+  constexpr int NW = NL2 > 1 ? NL2 - 1 : 1;
+  constexpr int P = 32 / EPW;  // partner rows per electron in a tile
+  __shared__ float xs[NCH][32][33];
+  const int lane = threadIdx.x, h = lane >> 5, fl = lane & 31;
+  const int n = sh.n, u = sh.u, C = sh.C, Cp = sh.Cp;
+  const int groups = (n + EPW - 1) / EPW;
+  const long b = blockIdx.x / groups;
+  const int i0 = (int)(blockIdx.x - b * groups) * EPW;  // first electron of the group
+  const float* rb = r + b * n * 3;
+  const float inv_up = 1.f / (float)u, inv_dn = 1.f / (float)(n - u);
+  const float rs2 = 0.70710678118654752440f;
+
+  // B operands: lane holds W[k = 2 s + h][f = fl]
+  float w0[2], wB[NW][16], bias[NL2 > 0 ? NL2 : 1];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) w0[q] = W2e[(2 * q + h) * 32 + fl];
+#pragma unroll
+  for (int l2 = 1; l2 < NL2; ++l2)
+#pragma unroll
+    for (int q = 0; q < 16; ++q) wB[l2 - 1][q] = W2e[(l2 * 32 + 2 * q + h) * 32 + fl];
+#pragma unroll
+  for (int l2 = 0; l2 < NL2; ++l2) bias[l2] = b2e[l2 * 32 + fl];
+
+  // per-lane spin sums of the layer outputs, per electron of the group
+  float su[NL2 > 0 ? NL2 : 1][EPW][NCH], sd[NL2 > 0 ? NL2 : 1][EPW][NCH];
+  float ru[NCH][2], rd[NCH][2];  // raw pair features of this lane's row (k = h, 2 + h)
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) {
+    ru[c][0] = ru[c][1] = rd[c][0] = rd[c][1] = 0.f;
+#pragma unroll
+    for (int l2 = 0; l2 < NL2; ++l2)
+#pragma unroll
+      for (int e = 0; e < EPW; ++e) su[l2][e][c] = sd[l2][e][c] = 0.f;
+  }
+  // this lane's A-layout row
+  const int eA = fl / P, iA = i0 + eA;
+
+  for (int jt = 0; jt < n; jt += P) {
+    // ---- layer-0 input in the A layout: row (eA, j = jt + fl % P), k-slots h and 2 + h
+    const int j = jt + fl % P;
+    const bool pair = (iA < n) && (j < n) && (j != iA);
+    float a0[NCH][2];
+    {
+      float dx = 0.f, dy = 0.f, dz = 0.f, rho = 1.f;
+      if (pair) {
+        dx = rb[j * 3 + 0] - rb[iA * 3 + 0];
+        dy = rb[j * 3 + 1] - rb[iA * 3 + 1];
+        dz = rb[j * 3 + 2] - rb[iA * 3 + 2];
+        rho = sqrtf(dx * dx + dy * dy + dz * dz);
+      }
+      const float dv[3] = {dx, dy, dz};
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int k = 2 * q + h;
+        float v = 0.f;
+        if (pair) v = (k == 0) ? rho : dv[k - 1];
+        a0[0][q] = v;
+        if constexpr (LAP) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) a0[1 + c][q] = pair ? ((k == 0) ? dv[c] / rho : ((k - 1 == c) ? 1.f : 0.f)) : 0.f;
+          a0[4][q] = (pair && k == 0) ? 4.f / rho : 0.f;
+        }
+      }
+    }
+    const bool up_j = j < u;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        if (j < n && up_j) ru[c][q] += a0[c][q];
+        if (j < n && !up_j) rd[c][q] += a0[c][q];
+      }
+    if constexpr (LAP) {
+      if (pair) {  // tangent rows of X_0
+        const float inv_s = up_j ? inv_up : inv_dn;
+        const int sj = up_j ? 0 : 1;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          float* row = out.X[0] + (((long)b * n + iA) * Cp + 1 + 3 * j + c) * (long)out.Kx[0] + out.Kh[0];
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            row[sj * 4 + 2 * q + h] = a0[1 + c][q] * inv_s;
+            row[(1 - sj) * 4 + 2 * q + h] = 0.f;
+          }
+        }
+      }
+    }
+
+    // ---- 2-el layers
+#pragma unroll
+    for (int l2 = 0; l2 < NL2; ++l2) {
+      v16f acc[NCH];
+#pragma unroll
+      for (int c = 0; c < NCH; ++c)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[c][q] = (c == 0) ? bias[l2] : 0.f;
+      if (l2 == 0) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+          for (int c = 0; c < NCH; ++c) acc[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[c][q], w0[q], acc[c], 0, 0, 0);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+#pragma unroll
+          for (int c = 0; c < NCH; ++c)
+            acc[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(xs[c][fl][2 * q + h], wB[l2 - 1][q], acc[c], 0, 0, 0);
+      }
+      // tanh with the forward Laplacian rule (in place), then the residual (x + y) / sqrt 2 on the
+      // 32 -> 32 layers with x read back from the transposition buffer (it still holds the layer input)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const float t = tanhf(acc[0][q]);
+        if constexpr (LAP) {
+          const float d1 = 1.f - t * t, d2 = -2.f * t * d1;
+          const float z1 = acc[1][q], z2 = acc[2][q], z3 = acc[3][q];
+          acc[4][q] = d1 * acc[4][q] + d2 * 2.f * (z1 * z1 + z2 * z2 + z3 * z3);
+          acc[1][q] = d1 * z1;
+          acc[2][q] = d1 * z2;
+          acc[3][q] = d1 * z3;
+        }
+        acc[0][q] = t;
+        if (l2 > 0) {
+#pragma unroll
+          for (int c = 0; c < NCH; ++c) acc[c][q] = (xs[c][mfma_row(q, h)][fl] + acc[c][q]) * rs2;
+        }
+      }
+      // spin sums and the tangent rows of X_{l2+1}
+      const long Kxo = out.Kx[l2 + 1];
+      const int Kho = out.Kh[l2 + 1];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int row = mfma_row(q, h);
+        const int e = row / P;  // compile-time per q and h when P >= 4
+        const int jr = jt + row % P;
+        const int ie = i0 + e;
+        if (jr < n && ie < n) {
+          const bool up = jr < u;
+#pragma unroll
+          for (int c = 0; c < NCH; ++c) {
+#pragma unroll
+            for (int e2 = 0; e2 < EPW; ++e2) {
+              if (e2 == e) {
+                if (up)
+                  su[l2][e2][c] += acc[c][q];
+                else
+                  sd[l2][e2][c] += acc[c][q];
+              }
+            }
+          }
+          if constexpr (LAP) {
+            if (jr != ie) {
+              const float inv_s = up ? inv_up : inv_dn;
+              const int sj = up ? 0 : 1;
+#pragma unroll
+              for (int c = 0; c < 3; ++c) {
+                float* rw = out.X[l2 + 1] + (((long)b * n + ie) * Cp + 1 + 3 * jr + c) * Kxo + Kho;
+                rw[sj * 32 + fl] = acc[1 + c][q] * inv_s;
+                rw[(1 - sj) * 32 + fl] = 0.f;
+              }
+            }
+          }
+        }
+      }
+      if (l2 + 1 < NL2) {  // transpose into the A layout of the next layer (wave-private LDS)
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+#pragma unroll
+          for (int c = 0; c < NCH; ++c) xs[c][mfma_row(q, h)][fl] = acc[c][q];
+        __syncthreads();
+      }
+    }
+    __syncthreads();  // xs is rewritten by the next row tile
+  }
+
+  // ---- means into the pair columns: half-wave h writes spin block h
+#pragma unroll
+  for (int l2 = 0; l2 < NL2; ++l2) {
+    float* X = out.X[l2 + 1];
+    const long Kx = out.Kx[l2 + 1];
+    const int col = out.Kh[l2 + 1] + h * 32 + fl;
+#pragma unroll
+    for (int e = 0; e < EPW; ++e) {
+      float m[NCH];
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        const float tu = su[l2][e][c] + __shfl_xor(su[l2][e][c], 32, 64);
+        const float td = sd[l2][e][c] + __shfl_xor(sd[l2][e][c], 32, 64);
+        m[c] = h == 0 ? tu * inv_up : td * inv_dn;
+      }
+      const int ie = i0 + e;
+      if (ie < n) {
+        const long base = ((long)b * n + ie) * Cp;
+        X[base * Kx + col] = m[0];
+        if constexpr (LAP) {
+          X[(base + C - 1) * Kx + col] = m[4];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) X[(base + 1 + 3 * ie + c) * Kx + col] = -m[1 + c];
+        }
+      }
+    }
+  }
+  // raw features: reduce over the P partner lanes of each electron (k = h, 2 + h)
+#pragma unroll
+  for (int c = 0; c < NCH; ++c)
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int o = P / 2; o > 0; o >>= 1) {
+        ru[c][q] += __shfl_xor(ru[c][q], o, 32);
+        rd[c][q] += __shfl_xor(rd[c][q], o, 32);
+      }
+  if (fl % P == 0 && iA < n) {
+    float* X = out.X[0];
+    const long Kx = out.Kx[0];
+    const int Kh = out.Kh[0];
+    const long base = ((long)b * n + iA) * Cp;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int k = 2 * q + h;
+#pragma unroll
+      for (int s2 = 0; s2 < 2; ++s2) {
+        const float inv = s2 == 0 ? inv_up : inv_dn;
+        X[base * Kx + Kh + s2 * 4 + k] = (s2 == 0 ? ru[0][q] : rd[0][q]) * inv;
+        if constexpr (LAP) {
+          X[(base + C - 1) * Kx + Kh + s2 * 4 + k] = (s2 == 0 ? ru[4][q] : rd[4][q]) * inv;
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+            X[(base + 1 + 3 * iA + c) * Kx + Kh + s2 * 4 + k] = -(s2 == 0 ? ru[1 + c][q] : rd[1 + c][q]) * inv;
+        }
+      }
+    }
+  }
+}
+
+template <bool LAP, int L>
+static void twoel_mfma_dispatch(const float* r, const TwoElOut<float>& o, const float* W2e, const float* b2e,
+                                const Shape& sh, hipStream_t s) {
+  // value pass: several electrons per wavefront for small n; the 5-channel Laplacian pass keeps one
+  // electron per wavefront (its per-electron sums would not fit in registers)
+  if constexpr (!LAP) {
+    const int epw = sh.n <= 8 ? 4 : (sh.n <= 16 ? 2 : 1);
+    const unsigned grid = (unsigned)(sh.B * ((sh.n + epw - 1) / epw));
+    if (epw == 4) {
+      hipLaunchKernelGGL((k_twoel_mfma<false, L, 4>), dim3(grid), dim3(64), 0, s, r, o, W2e, b2e, sh);
+      return;
+    }
+    if (epw == 2) {
+      hipLaunchKernelGGL((k_twoel_mfma<false, L, 2>), dim3(grid), dim3(64), 0, s, r, o, W2e, b2e, sh);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_twoel_mfma<LAP, L, 1>), dim3((unsigned)(sh.B * sh.n)), dim3(64), 0, s, r, o, W2e, b2e, sh);
+}
+
 template <bool LAP, typename T>
 static void twoel_dispatch(const float* r, const TwoElOut<T>& o, const T* W2e, const T* b2e, const Shape& sh,
                            hipStream_t s) {
+  if constexpr (sizeof(T) == 4) {
+    if (sh.W2 == 32) {
+      switch (sh.L) {
+        case 2: twoel_mfma_dispatch<LAP, 2>(r, o, W2e, b2e, sh, s); break;
+        case 3: twoel_mfma_dispatch<LAP, 3>(r, o, W2e, b2e, sh, s); break;
+        default: twoel_mfma_dispatch<LAP, 4>(r, o, W2e, b2e, sh, s); break;
+      }
+      return;
+    }
+  }
   const unsigned grid = (unsigned)(sh.B * ((sh.n + 7) / 8));
   switch (sh.L) {
     case 2: hipLaunchKernelGGL((k_twoel<LAP, 2, T>), dim3(grid), dim3(256), 0, s, r, o, W2e, b2e, sh); break;
@@ -283,28 +568,45 @@ template void launch_twoel<double>(const float*, double* const*, const int*, con
 // ------------------------------------------------------------------------------------------------
 template <typename T>
 __global__ void k_spin_mean(const T* __restrict__ X, int Kx, int Kh, T* __restrict__ MU, int Km, Shape sh) {
-  const long total = (long)sh.B * sh.Cp * Km;
+  // one thread per (walker, channel, 4 consecutive features); Kh, Kx and Km are multiples of 4
+  const int Km4 = Km >> 2;
+  const long total = (long)sh.B * sh.Cp * Km4;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
-  const int f = (int)(t % Km);
-  const long row = t / Km;
+  const long row = t / Km4;
+  const int f = (int)(t - row * Km4) * 4;
   const int ch = (int)(row % sh.Cp);
   const long b = row / sh.Cp;
-  T v = T(0);
+  T v0 = T(0), v1 = T(0), v2 = T(0), v3 = T(0);
   if (f < 2 * Kh) {
     const int s = f >= Kh ? 1 : 0;
     const int fi = f - s * Kh;
     const int i0 = s ? sh.u : 0, i1 = s ? sh.n : sh.u;
-    T acc = T(0);
-    for (int i = i0; i < i1; ++i) acc += X[((b * sh.n + i) * sh.Cp + ch) * (long)Kx + fi];
-    v = acc / (T)(i1 - i0);
+    const T* xp = X + ((b * sh.n + i0) * sh.Cp + ch) * (long)Kx + fi;
+    const long stride = (long)sh.Cp * Kx;
+#pragma unroll 4
+    for (int i = i0; i < i1; ++i, xp += stride) {
+      v0 += xp[0];
+      v1 += xp[1];
+      v2 += xp[2];
+      v3 += xp[3];
+    }
+    const T inv = T(1) / (T)(i1 - i0);
+    v0 *= inv;
+    v1 *= inv;
+    v2 *= inv;
+    v3 *= inv;
   }
-  MU[t] = v;
+  T* mp = MU + row * Km + f;
+  mp[0] = v0;
+  mp[1] = v1;
+  mp[2] = v2;
+  mp[3] = v3;
 }
 
 template <typename T>
 void launch_spin_mean(const T* X, int Kx, int Kh, T* MU, int Km, const Shape& sh, hipStream_t s) {
-  const long total = (long)sh.B * sh.Cp * Km;
+  const long total = (long)sh.B * sh.Cp * (Km / 4);
   hipLaunchKernelGGL(k_spin_mean<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, X, Kx, Kh, MU, Km, sh);
 }
 template void launch_spin_mean<float>(const float*, int, int, float*, int, const Shape&, hipStream_t);
@@ -398,12 +700,23 @@ __global__ void k_envelope(const T* __restrict__ O, int ldo, const float* __rest
   const T bf0 = O[orow * ldo + col];
   Ab[0] = env * bf0;
   if (C > 1) {
-    for (int ch = 1; ch < C - 1; ++ch) {
-      const int e = (ch - 1) / 3, c = (ch - 1) % 3;
-      T v = env * O[(orow + ch) * ldo + col];
-      if (e == i) v += (c == 0 ? g0 : (c == 1 ? g1 : g2)) * bf0;
-      Ab[(long)ch * nn] = v;
+    // tangent channels: loads of 8 channels are issued before their stores
+    const T* Oc = O + (orow + 1) * ldo + col;
+    T* Ac = Ab + nn;
+    const int nt = C - 2;
+    int ch = 0;
+    for (; ch + 8 <= nt; ch += 8) {
+      T v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = env * Oc[(long)(ch + q) * ldo];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) Ac[(long)(ch + q) * nn] = v[q];
     }
+    for (; ch < nt; ++ch) Ac[(long)ch * nn] = env * Oc[(long)ch * ldo];
+    // electron i's own coordinates also carry the envelope gradient
+    Ac[(long)(3 * i + 0) * nn] = env * Oc[(long)(3 * i + 0) * ldo] + g0 * bf0;
+    Ac[(long)(3 * i + 1) * nn] = env * Oc[(long)(3 * i + 1) * ldo] + g1 * bf0;
+    Ac[(long)(3 * i + 2) * nn] = env * Oc[(long)(3 * i + 2) * ldo] + g2 * bf0;
     const T t0 = O[(orow + 1 + 3 * i) * ldo + col];
     const T t1 = O[(orow + 2 + 3 * i) * ldo + col];
     const T t2 = O[(orow + 3 + 3 * i) * ldo + col];
@@ -1107,13 +1420,115 @@ __global__ void k_combine(const double* __restrict__ ld, const double* __restric
   if (out_flags) out_flags[b] = fl;
 }
 
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Laplacian pass: one wavefront per walker.  Lanes split the 3n gradient components and the electrons
+// of the potential; partial sums meet in a fixed shuffle tree (deterministic).  Same formulas as
+// k_combine<true>.
+__global__ __launch_bounds__(256) void k_combine_lap(const double* __restrict__ ld, const double* __restrict__ sd,
+                                                     const double* __restrict__ g, const double* __restrict__ lapd,
+                                                     const float* __restrict__ dcond, float cond_thr,
+                                                     const float* __restrict__ r, const float* __restrict__ R,
+                                                     const float* __restrict__ Z, float* __restrict__ out_lp,
+                                                     float* __restrict__ out_phase, float* __restrict__ out_eloc,
+                                                     uint32_t* __restrict__ out_flags, float* __restrict__ out_kappa,
+                                                     Shape sh) {
+  const int lane = threadIdx.x & 63;
+  const long b = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (b >= sh.B) return;  // uniform per wavefront
+  const int D = sh.D, n = sh.n, nd = 3 * n;
+  const double* l = ld + b * D;
+  const double* sg = sd + b * D;
+  double m = -INFINITY;
+  for (int d = 0; d < D; ++d) m = fmax(m, l[d]);
+  double S = 0.0, Sabs = 0.0, cmax = 0.0, lapm = 0.0;
+  int cnt = 0;
+  for (int d = 0; d < D; ++d) {
+    const double e = sg[d] * exp(l[d] - m);
+    S += e;
+    Sabs += fabs(e);
+    if (l[d] == m) {
+      ++cnt;
+      lapm += lapd[b * D + d];
+    }
+    cmax = fmax(cmax, (double)dcond[b * D + d]);
+  }
+  lapm /= cnt;
+  const double kappa = cmax * Sabs / fabs(S);
+  uint32_t fl = 0;
+  if (cond_thr > 0.f && !(kappa <= (double)cond_thr)) fl |= DWF_FP64;
+  const double eps = 1e-8;
+  const double aS = fabs(S);
+  const double fval = log(aS + eps) + m;
+  const float lp = (float)(2.0 * fval);
+  if (!isfinite(lp)) fl |= DWF_NONFINITE;
+  const double sgnS = (S > 0.0) ? 1.0 : ((S < 0.0) ? -1.0 : 0.0);
+  const double den = aS + eps;
+  const double* gb = g + b * D * nd;
+  double p_d2S = 0.0, p_dS2 = 0.0, p_df2 = 0.0;
+  for (int k = lane; k < nd; k += 64) {
+    double dm = 0.0;
+    for (int d = 0; d < D; ++d)
+      if (l[d] == m) dm += gb[d * nd + k];
+    dm /= cnt;
+    double dS = 0.0, d2 = 0.0;
+    for (int d = 0; d < D; ++d) {
+      const double e = sg[d] * exp(l[d] - m);
+      const double x = gb[d * nd + k] - dm;
+      dS += e * x;
+      d2 += e * x * x;
+    }
+    p_d2S += d2;
+    p_dS2 += dS * dS;
+    const double df = sgnS * dS / den + dm;
+    p_df2 += df * df;
+  }
+  // potential: electron pairs and electron-ion terms over lanes, ion-ion on every lane
+  const float* rb = r + b * n * 3;
+  double pe = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    const double xi = rb[i * 3], yi = rb[i * 3 + 1], zi = rb[i * 3 + 2];
+    for (int j = i + 1; j < n; ++j) {
+      const double dx = xi - rb[j * 3], dy = yi - rb[j * 3 + 1], dz = zi - rb[j * 3 + 2];
+      pe += 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    for (int I = 0; I < sh.I; ++I) {
+      const double dx = xi - R[I * 3], dy = yi - R[I * 3 + 1], dz = zi - R[I * 3 + 2];
+      pe -= (double)Z[I] / sqrt(dx * dx + dy * dy + dz * dz);
+    }
+  }
+  double sum_d2S = wave_sum_d(p_d2S);
+  const double sum_dS2 = wave_sum_d(p_dS2), sum_df2 = wave_sum_d(p_df2);
+  double epot = wave_sum_d(pe);
+  if (lane != 0) return;
+  for (int d = 0; d < D; ++d) sum_d2S += sg[d] * exp(l[d] - m) * (lapd[b * D + d] - lapm);
+  for (int I = 0; I < sh.I; ++I)
+    for (int J = I + 1; J < sh.I; ++J) {
+      const double dx = R[I * 3] - R[J * 3], dy = R[I * 3 + 1] - R[J * 3 + 1], dz = R[I * 3 + 2] - R[J * 3 + 2];
+      epot += (double)Z[I] * Z[J] / sqrt(dx * dx + dy * dy + dz * dz);
+    }
+  const double lapf = sgnS * sum_d2S / den - sum_dS2 / (den * den) + lapm;
+  const double ekin = -0.5 * (lapf + sum_df2);
+  const float eloc = (float)(ekin + epot);
+  if (!isfinite(eloc)) fl |= DWF_NONFINITE;
+  if (out_kappa) out_kappa[b] = (float)kappa;
+  if (out_lp) out_lp[b] = lp;
+  if (out_phase) out_phase[b] = (S < 0.0) ? 3.14159265358979323846f : 0.f;
+  if (out_eloc) out_eloc[b] = eloc;
+  if (out_flags) out_flags[b] = fl;
+}
+
 void launch_combine(const double* ld, const double* sd, const double* g, const double* lapd, const float* dcond,
                     float cond_thr, const float* r, const float* R, const float* Z, float* out_lp, float* out_phase,
                     float* out_eloc, uint32_t* out_flags, float* out_kappa, const Shape& sh, hipStream_t s) {
   const unsigned grid = (unsigned)((sh.B + 127) / 128);
   if (sh.C > 1)
-    hipLaunchKernelGGL(k_combine<true>, dim3(grid), dim3(128), 0, s, ld, sd, g, lapd, dcond, cond_thr, r, R, Z, out_lp,
-                       out_phase, out_eloc, out_flags, out_kappa, sh);
+    hipLaunchKernelGGL(k_combine_lap, dim3((unsigned)((sh.B + 3) / 4)), dim3(256), 0, s, ld, sd, g, lapd, dcond,
+                       cond_thr, r, R, Z, out_lp, out_phase, out_eloc, out_flags, out_kappa, sh);
   else
     hipLaunchKernelGGL(k_combine<false>, dim3(grid), dim3(128), 0, s, ld, sd, g, lapd, dcond, cond_thr, r, R, Z,
                        out_lp, out_phase, out_eloc, out_flags, out_kappa, sh);
@@ -1172,32 +1587,45 @@ void launch_propose(const float* r, const float* normals, const void* chain, flo
 
 // One 64-thread group per walker: lane 0 decides, all lanes copy the accepted coordinates.
 // accept = exp(lp' - lp) > u  or  old age >= max_age;  age <- 0 / age + 1;  lp and r selected.
-__global__ void k_accept(float* __restrict__ r, const float* __restrict__ r_prop, float* __restrict__ lp,
-                         const float* __restrict__ lp_prop, const float* __restrict__ uniforms,
-                         int32_t* __restrict__ age, int max_age, int32_t* __restrict__ acc_count,
-                         uint32_t* __restrict__ flags, int B, int n) {
-  const int b = blockIdx.x;
-  if (b >= B) return;
-  const int lane = threadIdx.x;
-  const float p_accept = expf(lp_prop[b] - lp[b]);
-  const bool by_ratio = p_accept > uniforms[b];
-  const int old_age = age[b];
-  const bool forced = old_age >= max_age;
-  const bool acc = by_ratio || forced;
-  for (int c = lane; c < n * 3; c += 64)
-    if (acc) r[(long)b * n * 3 + c] = r_prop[(long)b * n * 3 + c];
+__global__ __launch_bounds__(256) void k_accept(float* __restrict__ r, const float* __restrict__ r_prop,
+                                                float* __restrict__ lp, const float* __restrict__ lp_prop,
+                                                const float* __restrict__ uniforms, int32_t* __restrict__ age,
+                                                int max_age, int32_t* __restrict__ acc_count,
+                                                uint32_t* __restrict__ flags, int B, int n) {
+  // 64 walkers per workgroup: wave 0 decides (one lane per walker), all waves copy accepted positions
+  __shared__ int acc_sh[64];
+  const int tid = threadIdx.x;
+  const long b0 = (long)blockIdx.x * 64;
+  if (tid < 64) {
+    const long b = b0 + tid;
+    bool acc = false;
+    if (b < B) {
+      const float p_accept = expf(lp_prop[b] - lp[b]);
+      const bool by_ratio = p_accept > uniforms[b];
+      const int old_age = age[b];
+      const bool forced = old_age >= max_age;
+      acc = by_ratio || forced;
+      age[b] = acc ? 0 : old_age + 1;
+      if (acc) lp[b] = lp_prop[b];
+      if (flags) flags[b] = (forced && !by_ratio) ? DWF_FORCED : 0u;
+    }
+    acc_sh[tid] = acc ? 1 : 0;
+    const unsigned long long bal = __ballot(acc);
+    if (tid == 0 && bal) atomicAdd(acc_count, (int)__popcll(bal));
+  }
   __syncthreads();
-  if (lane == 0) {
-    age[b] = acc ? 0 : old_age + 1;
-    if (acc) lp[b] = lp_prop[b];
-    if (acc) atomicAdd(acc_count, 1);
-    if (flags) flags[b] = (forced && !by_ratio) ? DWF_FORCED : 0u;
+  const int per = 3 * n;
+  const long nw = (B - b0) < 64 ? (B - b0) : 64;
+  const long base = b0 * per;
+  for (long t = tid; t < nw * per; t += 256) {
+    const int w = (int)(t / per);
+    if (acc_sh[w]) r[base + t] = r_prop[base + t];
   }
 }
 void launch_accept(float* r, const float* r_prop, float* lp, const float* lp_prop, const float* uniforms,
                    int32_t* age, int max_age, int32_t* acc_count, uint32_t* flags, int B, int n, hipStream_t s) {
-  hipLaunchKernelGGL(k_accept, dim3(B), dim3(64), 0, s, r, r_prop, lp, lp_prop, uniforms, age, max_age, acc_count,
-                     flags, B, n);
+  hipLaunchKernelGGL(k_accept, dim3((unsigned)((B + 63) / 64)), dim3(256), 0, s, r, r_prop, lp, lp_prop, uniforms,
+                     age, max_age, acc_count, flags, B, n);
 }
 
 // acc = mean(accept); step_nr += 1; acc_rate = 0.9 acc_rate + 0.1 acc; every `interval` steps the
