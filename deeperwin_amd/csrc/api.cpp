@@ -432,7 +432,7 @@ dw_status run_pass(dw_model* m, const float* r, long B, const float* R, const fl
         const long tiles64 = ((B * Cp + 63) / 64) * ((m->W1 + 63) / 64);
         vs = tiles64 >= 512 ? 1 : (tiles64 >= 256 ? 2 : 4);
       }
-      while (vs > 1 && m->Km[l] % (16 * vs) != 0) vs >>= 1;
+      while (vs > 1 && m->Km[l] % (16 * vs) != 0) vs >>= 1;  // each K slice a multiple of the 16-wide K tile
     }
     GemmEpiT<T> ev;
     ev.bias = W.bias[l].as<T>();

@@ -664,14 +664,19 @@ template void launch_tanh_rule<double>(const double*, double*, int, int, const S
 template <typename T>
 __global__ void k_envelope(const T* __restrict__ O, int ldo, const float* __restrict__ r, const float* __restrict__ R,
                            const T* __restrict__ kappa, const T* __restrict__ wenv, T* __restrict__ A, Shape sh) {
+  // thread order (b, d, i, k): consecutive threads write consecutive elements of A for every channel
   const int n = sh.n, NO = sh.D * n, C = sh.C;
   const long total = (long)sh.B * n * NO;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
-  const int col = (int)(t % NO);
-  const long bi = t / NO;
-  const int i = (int)(bi % n);
-  const long b = bi / n;
+  const int nn_i = n * n;
+  const long bd_ = t / nn_i;
+  const int ik = (int)(t - bd_ * nn_i);
+  const int i = ik / n, kk_ = ik - (ik / n) * n;
+  const long b = bd_ / sh.D;
+  const int d_ = (int)(bd_ - b * sh.D);
+  const int col = d_ * n + kk_;
+  const long bi = b * n + i;
   const int s = (i < sh.u) ? 0 : 1;
   const int dn = n - sh.u;
   const int Cp = sh.Cp;
@@ -933,7 +938,8 @@ __global__ __launch_bounds__(64 * WPB) void k_det_lap(const TA* __restrict__ A, 
   long bd = (long)blockIdx.x * WPB + wv;
   const bool valid_mat = bd < nmat;
   if (!valid_mat) bd = nmat - 1;  // keep the wave in the barriers; results are not stored
-  double* sMi = dsm + (size_t)wv * (NMAX * NMAX + 64 * NMAX);
+  constexpr int SYW = (NMAX == 16) ? 0 : 64 * NMAX;  // the MFMA channel passes (NMAX 16) need no Y buffer
+  double* sMi = dsm + (size_t)wv * (NMAX * NMAX + SYW);
   double* sY = sMi + NMAX * NMAX;
   const long nn = (long)n * n;
   const TA* Ab = A + bd * C * nn;
@@ -989,54 +995,87 @@ __global__ __launch_bounds__(64 * WPB) void k_det_lap(const TA* __restrict__ A, 
   }
   __syncthreads();
 
-  const int cg = lane / NMAX, c = lane % NMAX;
-  const bool cact = c < n;
   double qacc = 0.0;
-  for (int k0 = 0; k0 < nd; k0 += CPP) {
-    const int k = k0 + cg;
-    const bool kact = k < nd;
-    const TA* Tg = Ab + (long)(1 + (kact ? k : 0)) * nn;
-    double jc[NMAX];
+  if constexpr (NMAX == 16) {
+    // Channel passes on FP64 MFMA (v_mfma_f64_16x16x4f64).  Lane l holds M^-1[l%16][l/16 + 4s] (A operand
+    // of Y_k = M^-1 dM_k) and dM_k[l/16 + 4s][l%16] (its B operand); the same registers are the B and A
+    // operands of Y_k^T = dM_k^T M^-T.  Both products come out in one layout, so tr Y_k^2 is the sum of
+    // their elementwise product, and tr Y_k is the sum of the operand products.
+    const int ri = lane & 15, kq = lane >> 4;
+    double miA[4];
 #pragma unroll
-    for (int e = 0; e < NMAX; ++e) jc[e] = (cact && kact && e < n) ? (double)Tg[e * n + c] : 0.0;
-    double yc[NMAX];
+    for (int q = 0; q < 4; ++q) miA[q] = sMi[ri * NMAX + kq + 4 * q];
+    const bool jin = ri < n;
+    for (int k = 0; k < nd; ++k) {
+      const TA* Tg = Ab + (long)(1 + k) * nn;
+      double dmB[4];
+      double gp = 0.0;
 #pragma unroll
-    for (int a = 0; a < NMAX; ++a) {
-      double acc = 0.0;
-#pragma unroll
-      for (int e = 0; e < NMAX; e += 2) {
-        const double2 m2 = *(const double2*)&sMi[a * NMAX + e];
-        acc += m2.x * jc[e] + m2.y * jc[e + 1];
+      for (int q = 0; q < 4; ++q) {
+        const int kk = kq + 4 * q;
+        dmB[q] = (jin && kk < n) ? (double)Tg[kk * n + ri] : 0.0;
+        gp += miA[q] * dmB[q];
       }
-      yc[a] = acc;
-    }
-    double diag = 0.0;
+      v4d y = {0.0, 0.0, 0.0, 0.0}, yt = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int a = 0; a < NMAX; ++a) diag += (a == c) ? yc[a] : 0.0;
-#pragma unroll
-    for (int a = 0; a < NMAX; ++a) sY[(cg * NMAX + a) * NMAX + c] = yc[a];
-    __syncthreads();
-    double qp = 0.0;
-#pragma unroll
-    for (int a = 0; a < NMAX; ++a) qp += sY[(cg * NMAX + c) * NMAX + a] * yc[a];
-    if (!cact) {
-      qp = 0.0;
-      diag = 0.0;
+      for (int q = 0; q < 4; ++q) {
+        y = __builtin_amdgcn_mfma_f64_16x16x4f64(miA[q], dmB[q], y, 0, 0, 0);
+        yt = __builtin_amdgcn_mfma_f64_16x16x4f64(dmB[q], miA[q], yt, 0, 0, 0);
+      }
+      qacc += y[0] * yt[0] + y[1] * yt[1] + y[2] * yt[2] + y[3] * yt[3];
+      gp = wave_sum_d(gp);
+      if (lane == 0 && valid_mat) g[bd * nd + k] = gp;
     }
-#pragma unroll
-    for (int o = NMAX / 2; o > 0; o >>= 1) {
-      qp += __shfl_xor(qp, o, NMAX);
-      diag += __shfl_xor(diag, o, NMAX);
+    qacc = wave_sum_d(qacc);
+  } else {
+    const int cg = lane / NMAX, c = lane % NMAX;
+    const bool cact = c < n;
+    for (int k0 = 0; k0 < nd; k0 += CPP) {
+      const int k = k0 + cg;
+      const bool kact = k < nd;
+      const TA* Tg = Ab + (long)(1 + (kact ? k : 0)) * nn;
+      double jc[NMAX];
+  #pragma unroll
+      for (int e = 0; e < NMAX; ++e) jc[e] = (cact && kact && e < n) ? (double)Tg[e * n + c] : 0.0;
+      double yc[NMAX];
+  #pragma unroll
+      for (int a = 0; a < NMAX; ++a) {
+        double acc = 0.0;
+  #pragma unroll
+        for (int e = 0; e < NMAX; e += 2) {
+          const double2 m2 = *(const double2*)&sMi[a * NMAX + e];
+          acc += m2.x * jc[e] + m2.y * jc[e + 1];
+        }
+        yc[a] = acc;
+      }
+      double diag = 0.0;
+  #pragma unroll
+      for (int a = 0; a < NMAX; ++a) diag += (a == c) ? yc[a] : 0.0;
+  #pragma unroll
+      for (int a = 0; a < NMAX; ++a) sY[(cg * NMAX + a) * NMAX + c] = yc[a];
+      __syncthreads();
+      double qp = 0.0;
+  #pragma unroll
+      for (int a = 0; a < NMAX; ++a) qp += sY[(cg * NMAX + c) * NMAX + a] * yc[a];
+      if (!cact) {
+        qp = 0.0;
+        diag = 0.0;
+      }
+  #pragma unroll
+      for (int o = NMAX / 2; o > 0; o >>= 1) {
+        qp += __shfl_xor(qp, o, NMAX);
+        diag += __shfl_xor(diag, o, NMAX);
+      }
+      if (c == 0 && kact) {
+        if (valid_mat) g[bd * nd + k] = diag;
+        qacc += qp;
+      }
+      __syncthreads();
     }
-    if (c == 0 && kact) {
-      if (valid_mat) g[bd * nd + k] = diag;
-      qacc += qp;
-    }
-    __syncthreads();
+    // sum q over the channel groups (group leaders hold partial sums)
+  #pragma unroll
+    for (int o = NMAX; o < 64; o <<= 1) qacc += __shfl_xor(qacc, o, 64);
   }
-  // sum q over the channel groups (group leaders hold partial sums)
-#pragma unroll
-  for (int o = NMAX; o < 64; o <<= 1) qacc += __shfl_xor(qacc, o, 64);
   // tr(M^-1 Lap M): lane c takes column c of Lap M and row c of M^-1
   const TA* Tl = Ab + (long)(C - 1) * nn;
   double tp = 0.0;
@@ -1305,7 +1344,7 @@ void launch_det(const TA* A, double* ld, double* sd, double* g, double* lapd, fl
   }
   if (n <= 16) {
     constexpr int NM = 16, W = 4;
-    const size_t shm = (size_t)W * (NM * NM + 64 * NM) * sizeof(double);
+    const size_t shm = (size_t)W * (NM * NM) * sizeof(double);
     hipLaunchKernelGGL((k_det_lap<NM, W, TA>), dim3((unsigned)((nmat + W - 1) / W)), dim3(64 * W), shm, s, A, ld, sd,
                        g, lapd, dcond, nmat, n, sh.C);
   } else {
@@ -1418,12 +1457,6 @@ __global__ void k_combine(const double* __restrict__ ld, const double* __restric
     if (out_eloc) out_eloc[b] = eloc;
   }
   if (out_flags) out_flags[b] = fl;
-}
-
-__device__ __forceinline__ double wave_sum_d(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
 }
 
 // Laplacian pass: one wavefront per walker.  Lanes split the 3n gradient components and the electrons
