@@ -17,7 +17,7 @@ namespace dwmi {
 
 enum { EPI_BIAS = 1, EPI_V = 2, EPI_TANH = 4, EPI_PERM = 8 };
 
-__device__ __forceinline__ float act_tanh(float v) { return tanhf(v); }
+__device__ __forceinline__ float act_tanh(float v) { return tanh_fast(v); }
 __device__ __forceinline__ double act_tanh(double v) { return tanh(v); }
 
 // Epilogue of one element at row r = (b * n + i) * C + ch, column c.  Index math in 32 bits (M < 2^31).
@@ -682,7 +682,7 @@ __global__ __launch_bounds__(256) void k_gemm_lap(const float* __restrict__ X, i
         else
           obase = g * Cp;
         Ez = &Es[e * Cp][c];
-        const float th = tanhf(Ez[0] + Vb[0]);
+        const float th = tanh_fast(Ez[0] + Vb[0]);
         d1 = 1.f - th * th;
         d2 = -2.f * th * d1;
         if (grp == 0) out[obase * ldo + gc] = th;

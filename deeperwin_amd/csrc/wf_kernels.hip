@@ -18,7 +18,7 @@ static constexpr int MAXL = 4;  // FermiNet iterations supported by the fused 2-
 // Scalar-type generic math (T = float for the FP32/BF16 forms, double for the FP64 form).
 __device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
 __device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ float dtanh(float x) { return tanhf(x); }
+__device__ __forceinline__ float dtanh(float x) { return tanh_fast(x); }
 __device__ __forceinline__ double dtanh(double x) { return tanh(x); }
 __device__ __forceinline__ float dexp(float x) { return expf(x); }
 __device__ __forceinline__ double dexp(double x) { return exp(x); }
@@ -379,7 +379,7 @@ __global__ __launch_bounds__(64) void k_twoel_mfma(const float* __restrict__ r, 
       // 32 -> 32 layers with x read back from the transposition buffer (it still holds the layer input)
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
-        const float t = tanhf(acc[0][q]);
+        const float t = tanh_fast(acc[0][q]);
         if constexpr (LAP) {
           const float d1 = 1.f - t * t, d2 = -2.f * t * d1;
           const float z1 = acc[1][q], z2 = acc[2][q], z3 = acc[3][q];
