@@ -28,6 +28,11 @@ __device__ __forceinline__ float tanh_fast(float x) {
   return 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + e);
 }
 
+// Index division: 32-bit unsigned arithmetic whenever the dividend fits (the common case), 64-bit otherwise.
+__device__ __forceinline__ long div_idx(long t, int d) {
+  return ((unsigned long)t >> 32) == 0 ? (long)((unsigned)t / (unsigned)d) : t / d;
+}
+
 __device__ __forceinline__ double wave_sum_d(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

@@ -34,11 +34,11 @@ __global__ void k_features0(const float* __restrict__ r, const float* __restrict
   const long total = (long)sh.B * sh.n * sh.Cp * Kx0;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
-  const int col = (int)(t % Kx0);
-  const long row = t / Kx0;
-  const int ch = (int)(row % sh.Cp);
-  const long bi = row / sh.Cp;
-  const int i = (int)(bi % sh.n);
+  const long row = div_idx(t, Kx0);
+  const int col = (int)(t - row * Kx0);
+  const long bi = div_idx(row, sh.Cp);
+  const int ch = (int)(row - bi * sh.Cp);
+  const int i = (int)(bi - div_idx(bi, sh.n) * sh.n);
   const int I4 = 4 * sh.I;
   if (ch >= sh.C) {  // padding channel rows stay zero
     X0[t] = T(0);
@@ -573,10 +573,10 @@ __global__ void k_spin_mean(const T* __restrict__ X, int Kx, int Kh, T* __restri
   const long total = (long)sh.B * sh.Cp * Km4;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
-  const long row = t / Km4;
+  const long row = div_idx(t, Km4);
   const int f = (int)(t - row * Km4) * 4;
-  const int ch = (int)(row % sh.Cp);
-  const long b = row / sh.Cp;
+  const long b = div_idx(row, sh.Cp);
+  const int ch = (int)(row - b * sh.Cp);
   T v0 = T(0), v1 = T(0), v2 = T(0), v3 = T(0);
   if (f < 2 * Kh) {
     const int s = f >= Kh ? 1 : 0;
@@ -670,10 +670,10 @@ __global__ void k_envelope(const T* __restrict__ O, int ldo, const float* __rest
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
   const int nn_i = n * n;
-  const long bd_ = t / nn_i;
+  const long bd_ = div_idx(t, nn_i);
   const int ik = (int)(t - bd_ * nn_i);
   const int i = ik / n, kk_ = ik - (ik / n) * n;
-  const long b = bd_ / sh.D;
+  const long b = div_idx(bd_, sh.D);
   const int d_ = (int)(bd_ - b * sh.D);
   const int col = d_ * n + kk_;
   const long bi = b * n + i;
@@ -812,9 +812,10 @@ __device__ __forceinline__ int lu_columns(double (&col)[NMAX], int (&perm)[NMAX]
     const double pivot = readlane_d(col[p], p);
     if (lane == p) {
       mydiag = col[p];
+      const double rp = 1.0 / pivot;  // multipliers by the reciprocal, as LAPACK's dgetf2 scales
 #pragma unroll
       for (int i = p + 1; i < NMAX; ++i)
-        if (i < n) col[i] = col[i] / pivot;
+        if (i < n) col[i] = col[i] * rp;
     }
 #pragma unroll
     for (int i = p + 1; i < NMAX; ++i) {
@@ -896,9 +897,10 @@ __global__ __launch_bounds__(256) void k_det_value(const TA* __restrict__ A, dou
       const double pivot = __shfl(col[p], base + p, 64);
       if (j == p) {
         mydiag = col[p];
+        const double rp = 1.0 / pivot;  // multipliers by the reciprocal, as LAPACK's dgetf2 scales
 #pragma unroll
         for (int i = p + 1; i < NMAX; ++i)
-          if (i < n) col[i] = col[i] / pivot;
+          if (i < n) col[i] = col[i] * rp;
       }
 #pragma unroll
       for (int i = p + 1; i < NMAX; ++i) {
