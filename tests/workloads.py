@@ -27,6 +27,15 @@ def system(name: str, n_det: int = None, seed: int = 0):
 
         R, Z = random_molecule(np.random.default_rng(60), 60)
         D = n_det or 2
+    elif name == "li":
+        R, Z = np.zeros((1, 3)), np.array([3])
+        D = n_det or 3
+    elif name in ("odd17", "odd33"):
+        from deeperwin_amd.physics import random_molecule
+
+        k = int(name[3:])
+        R, Z = random_molecule(np.random.default_rng(k), k)
+        D = n_det or 3
     elif name == "he":
         R, Z = np.zeros((1, 3)), np.array([2])
         D = n_det or 2
