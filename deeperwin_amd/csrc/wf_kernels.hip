@@ -928,21 +928,20 @@ __global__ __launch_bounds__(256) void k_det_value(const TA* __restrict__ A, dou
   }
 }
 
-// Laplacian mode: WPB matrices per workgroup, LDS holds M^-1 and the per-pass Y_k of every wave.
+// Laplacian mode (n <= 32): WPB matrices per workgroup, one per wavefront; LU and inverse with a lane per
+// column, M^-1 through LDS into MFMA operand registers, then the channel passes on FP64 MFMA.
 template <int NMAX, int WPB, typename TA>
 __global__ __launch_bounds__(64 * WPB) void k_det_lap(const TA* __restrict__ A, double* __restrict__ ld,
                                                       double* __restrict__ sd, double* __restrict__ g,
                                                       double* __restrict__ lapd, float* __restrict__ dcond, long nmat,
                                                       int n, int C) {
-  constexpr int CPP = 64 / NMAX;  // channels per pass
   extern __shared__ double dsm[];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   long bd = (long)blockIdx.x * WPB + wv;
   const bool valid_mat = bd < nmat;
   if (!valid_mat) bd = nmat - 1;  // keep the wave in the barriers; results are not stored
-  constexpr int SYW = (NMAX == 16) ? 0 : 64 * NMAX;  // the MFMA channel passes (NMAX 16) need no Y buffer
-  double* sMi = dsm + (size_t)wv * (NMAX * NMAX + SYW);
-  double* sY = sMi + NMAX * NMAX;
+  static_assert(NMAX == 16 || NMAX == 32, "channel passes are written for 16 x 16 MFMA blocks");
+  double* sMi = dsm + (size_t)wv * (NMAX * NMAX);
   const long nn = (long)n * n;
   const TA* Ab = A + bd * C * nn;
   const bool act = lane < n;
@@ -1029,54 +1028,55 @@ __global__ __launch_bounds__(64 * WPB) void k_det_lap(const TA* __restrict__ A, 
       if (lane == 0 && valid_mat) g[bd * nd + k] = gp;
     }
     qacc = wave_sum_d(qacc);
-  } else {
-    const int cg = lane / NMAX, c = lane % NMAX;
-    const bool cact = c < n;
-    for (int k0 = 0; k0 < nd; k0 += CPP) {
-      const int k = k0 + cg;
-      const bool kact = k < nd;
-      const TA* Tg = Ab + (long)(1 + (kact ? k : 0)) * nn;
-      double jc[NMAX];
-  #pragma unroll
-      for (int e = 0; e < NMAX; ++e) jc[e] = (cact && kact && e < n) ? (double)Tg[e * n + c] : 0.0;
-      double yc[NMAX];
-  #pragma unroll
-      for (int a = 0; a < NMAX; ++a) {
-        double acc = 0.0;
-  #pragma unroll
-        for (int e = 0; e < NMAX; e += 2) {
-          const double2 m2 = *(const double2*)&sMi[a * NMAX + e];
-          acc += m2.x * jc[e] + m2.y * jc[e + 1];
+  } else if constexpr (NMAX == 32) {
+    // The same on 2 x 2 blocks of 16: lane l holds M^-1[I*16 + l%16][K*16 + l/16 + 4s] and
+    // dM_k[K*16 + l/16 + 4s][J*16 + l%16] for all blocks; Y_IJ = sum_K M^-1_IK dM_KJ and
+    // (Y^T)_IJ = sum_K (dM_KI)^T (M^-1_JK)^T reuse the same registers.
+    const int ri = lane & 15, kq = lane >> 4;
+    double miA[2][2][4];
+#pragma unroll
+    for (int I = 0; I < 2; ++I)
+#pragma unroll
+      for (int K = 0; K < 2; ++K)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) miA[I][K][q] = sMi[(I * 16 + ri) * NMAX + K * 16 + kq + 4 * q];
+    for (int k = 0; k < nd; ++k) {
+      const TA* Tg = Ab + (long)(1 + k) * nn;
+      double dmB[2][2][4];
+#pragma unroll
+      for (int K = 0; K < 2; ++K)
+#pragma unroll
+        for (int J = 0; J < 2; ++J)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int rr = K * 16 + kq + 4 * q, cc = J * 16 + ri;
+            dmB[K][J][q] = (rr < n && cc < n) ? (double)Tg[rr * n + cc] : 0.0;
+          }
+      double gp = 0.0;
+#pragma unroll
+      for (int I = 0; I < 2; ++I)
+#pragma unroll
+        for (int K = 0; K < 2; ++K)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) gp += miA[I][K][q] * dmB[K][I][q];
+#pragma unroll
+      for (int I = 0; I < 2; ++I)
+#pragma unroll
+        for (int J = 0; J < 2; ++J) {
+          v4d y = {0.0, 0.0, 0.0, 0.0}, yt = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int K = 0; K < 2; ++K)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              y = __builtin_amdgcn_mfma_f64_16x16x4f64(miA[I][K][q], dmB[K][J][q], y, 0, 0, 0);
+              yt = __builtin_amdgcn_mfma_f64_16x16x4f64(dmB[K][I][q], miA[J][K][q], yt, 0, 0, 0);
+            }
+          qacc += y[0] * yt[0] + y[1] * yt[1] + y[2] * yt[2] + y[3] * yt[3];
         }
-        yc[a] = acc;
-      }
-      double diag = 0.0;
-  #pragma unroll
-      for (int a = 0; a < NMAX; ++a) diag += (a == c) ? yc[a] : 0.0;
-  #pragma unroll
-      for (int a = 0; a < NMAX; ++a) sY[(cg * NMAX + a) * NMAX + c] = yc[a];
-      __syncthreads();
-      double qp = 0.0;
-  #pragma unroll
-      for (int a = 0; a < NMAX; ++a) qp += sY[(cg * NMAX + c) * NMAX + a] * yc[a];
-      if (!cact) {
-        qp = 0.0;
-        diag = 0.0;
-      }
-  #pragma unroll
-      for (int o = NMAX / 2; o > 0; o >>= 1) {
-        qp += __shfl_xor(qp, o, NMAX);
-        diag += __shfl_xor(diag, o, NMAX);
-      }
-      if (c == 0 && kact) {
-        if (valid_mat) g[bd * nd + k] = diag;
-        qacc += qp;
-      }
-      __syncthreads();
+      gp = wave_sum_d(gp);
+      if (lane == 0 && valid_mat) g[bd * nd + k] = gp;
     }
-    // sum q over the channel groups (group leaders hold partial sums)
-  #pragma unroll
-    for (int o = NMAX; o < 64; o <<= 1) qacc += __shfl_xor(qacc, o, 64);
+    qacc = wave_sum_d(qacc);
   }
   // tr(M^-1 Lap M): lane c takes column c of Lap M and row c of M^-1
   const TA* Tl = Ab + (long)(C - 1) * nn;
@@ -1260,40 +1260,55 @@ __global__ __launch_bounds__(256) void k_det_lds(const TA* __restrict__ A, doubl
       nI = fmax(nI, lcol[8 + w]);
     }
   }
+  // Channel passes on FP64 MFMA over 16 x 16 blocks (nb = ceil(n / 16) <= 4): wave J computes the block
+  // column J of Y_k = M^-1 dM_k and the matching blocks of Y_k^T = dM_k^T M^-T, so tr Y_k^2 is the sum of
+  // their elementwise products; lane l holds operand elements [l%16][l/16 + 4s] (A) / [l/16 + 4s][l%16] (B).
   const int nd = 3 * n;
-  const int npair = n * (n + 1) / 2;
+  const int nb = (n + 15) >> 4;
+  const int J = wid, ri = lane & 15, kq = lane >> 4;
+  const bool wact = J < nb;
+  auto mi_at = [&](int rr, int cc) -> double { return (rr < n && cc < n) ? Mi[rr * P + cc] : 0.0; };
+  auto dm_at = [&](int rr, int cc) -> double { return (rr < n && cc < n) ? T[rr * P + cc] : 0.0; };
+  double miJ[4][4];  // M^-1[J*16 + ri][K*16 + kq + 4 q]
+#pragma unroll
+  for (int K = 0; K < 4; ++K)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) miJ[K][q] = wact ? mi_at(J * 16 + ri, K * 16 + kq + 4 * q) : 0.0;
   double qacc = 0.0;
   for (int ch = 1; ch <= nd; ++ch) {
     __syncthreads();
     const TA* Tg = Ab + (long)ch * nn;
     for (int e = tid; e < n * n; e += 256) T[(e / n) * P + (e % n)] = (double)Tg[e];
     __syncthreads();
-    double pd = 0.0, pq = 0.0;
-    for (int t = tid; t < npair; t += 256) {
-      // t -> (a, c) with a <= c
-      int a = 0, rem = t;
-      while (rem >= n - a) {
-        rem -= n - a;
-        ++a;
-      }
-      const int c = a + rem;
-      double xac = 0.0, xca = 0.0;
-      for (int q = 0; q < n; ++q) {
-        xac += Mi[a * P + q] * T[q * P + c];
-        xca += Mi[c * P + q] * T[q * P + a];
-      }
-      if (a == c) {
-        pd += xac;
-        pq += xac * xac;
-      } else {
-        pq += 2.0 * xac * xca;
+    double pd = 0.0;
+    if (wact) {
+      double dmJ[4][4];  // dM[K*16 + kq + 4 q][J*16 + ri]
+#pragma unroll
+      for (int K = 0; K < 4; ++K)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          dmJ[K][q] = (K < nb) ? dm_at(K * 16 + kq + 4 * q, J * 16 + ri) : 0.0;
+          pd += miJ[K][q] * dmJ[K][q];
+        }
+      for (int I = 0; I < nb; ++I) {
+        v4d y = {0.0, 0.0, 0.0, 0.0}, yt = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int K = 0; K < 4; ++K) {
+          if (K < nb) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              y = __builtin_amdgcn_mfma_f64_16x16x4f64(mi_at(I * 16 + ri, K * 16 + kq + 4 * q), dmJ[K][q], y, 0, 0, 0);
+              yt = __builtin_amdgcn_mfma_f64_16x16x4f64(dm_at(K * 16 + kq + 4 * q, I * 16 + ri), miJ[K][q], yt, 0, 0, 0);
+            }
+          }
+        }
+        qacc += y[0] * yt[0] + y[1] * yt[1] + y[2] * yt[2] + y[3] * yt[3];
       }
     }
     const double gs = block_sum_d(pd, red);
-    const double qs = block_sum_d(pq, red + 8);
     if (tid == 0) g[bd * nd + (ch - 1)] = gs;
-    qacc += qs;
   }
+  qacc = block_sum_d(qacc, red + 8);
   __syncthreads();
   const TA* Tl = Ab + (long)(C - 1) * nn;
   double tp = 0.0;
@@ -1351,7 +1366,7 @@ void launch_det(const TA* A, double* ld, double* sd, double* g, double* lapd, fl
                        g, lapd, dcond, nmat, n, sh.C);
   } else {
     constexpr int NM = 32, W = 2;
-    const size_t shm = (size_t)W * (NM * NM + 64 * NM) * sizeof(double);
+    const size_t shm = (size_t)W * (NM * NM) * sizeof(double);
     hipLaunchKernelGGL((k_det_lap<NM, W, TA>), dim3((unsigned)((nmat + W - 1) / W)), dim3(64 * W), shm, s, A, ld, sd,
                        g, lapd, dcond, nmat, n, sh.C);
   }
