@@ -1325,6 +1325,68 @@ __global__ __launch_bounds__(256) void k_det_lds(const TA* __restrict__ A, doubl
   }
 }
 
+// Value mode, 32 < n <= 64: one wavefront per matrix, the matrix in LDS (FP64, pitch n + 1), lane j owns
+// column j.  Partial-pivot LU: argmax of the pivot column by shuffles (lowest row on ties), row swap and
+// rank-1 update column-parallel; a single-wavefront workgroup, so each barrier only orders the LDS
+// traffic of one wavefront.
+template <typename TA>
+__global__ __launch_bounds__(64) void k_det_value_lds(const TA* __restrict__ A, double* __restrict__ ld,
+                                                      double* __restrict__ sd, float* __restrict__ dcond, int n,
+                                                      long stride) {
+  extern __shared__ double tm[];
+  const int P = n + 1, j = threadIdx.x;
+  const long bd = blockIdx.x;
+  const TA* Ab = A + bd * stride;
+  for (int e = j; e < n * n; e += 64) tm[(e / n) * P + (e % n)] = (double)Ab[e];
+  __syncthreads();
+  int swaps = 0;
+  bool sing = false;
+  double lg = 0.0, umax = 0.0, umin = 1e300;
+  int neg = 0;
+  for (int p = 0; p < n; ++p) {
+    double best = (j >= p && j < n) ? fabs(tm[j * P + p]) : -1.0;
+    int bidx = (j >= p && j < n) ? j : n;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ob = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bidx, o, 64);
+      if (ob > best || (ob == best && oi < bidx)) {
+        best = ob;
+        bidx = oi;
+      }
+    }
+    if (!(best > 0.0)) sing = true;
+    if (bidx != p && bidx < n) {
+      ++swaps;
+      if (j < n) {
+        const double t = tm[p * P + j];
+        tm[p * P + j] = tm[bidx * P + j];
+        tm[bidx * P + j] = t;
+      }
+    }
+    __syncthreads();
+    const double pivot = tm[p * P + p];
+    const double ap = fabs(pivot);
+    lg += log(ap);
+    neg += pivot < 0.0 ? 1 : 0;
+    umax = fmax(umax, ap);
+    umin = fmin(umin, ap);
+    const double rp = 1.0 / pivot;  // multipliers by the reciprocal, as LAPACK's dgetf2 scales
+    if (j > p && j < n) tm[j * P + p] *= rp;
+    __syncthreads();
+    if (j > p && j < n) {
+      const double up = tm[p * P + j];
+      for (int i = p + 1; i < n; ++i) tm[i * P + j] -= tm[i * P + p] * up;
+    }
+    __syncthreads();
+  }
+  if (j == 0) {
+    ld[bd] = sing ? -INFINITY : lg;
+    sd[bd] = sing ? 0.0 : (((neg + swaps) & 1) ? -1.0 : 1.0);
+    dcond[bd] = (float)(umax / umin);
+  }
+}
+
 template <typename TA>
 void launch_det(const TA* A, double* ld, double* sd, double* g, double* lapd, float* dcond, const Shape& sh,
                 hipStream_t s) {
@@ -1334,10 +1396,8 @@ void launch_det(const TA* A, double* ld, double* sd, double* g, double* lapd, fl
   if (n > 32) {
     const size_t shm = (size_t)(2 * n * (n + 1) + 64 + 32) * sizeof(double) + 64 * sizeof(int);
     if (sh.C == 1) {
-      if (shm > 65536) (void)hipFuncSetAttribute((const void*)k_det_lds<TA, false>,
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-      hipLaunchKernelGGL((k_det_lds<TA, false>), dim3((unsigned)nmat), dim3(256), shm, s, A, ld, sd, g, lapd, dcond, n,
-                         sh.C);
+      const size_t shv = (size_t)n * (n + 1) * sizeof(double);  // <= 31.5 KB for n <= 63
+      hipLaunchKernelGGL((k_det_value_lds<TA>), dim3((unsigned)nmat), dim3(64), shv, s, A, ld, sd, dcond, n, stride);
     } else {
       if (shm > 65536) (void)hipFuncSetAttribute((const void*)k_det_lds<TA, true>,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
