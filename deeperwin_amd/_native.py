@@ -55,6 +55,7 @@ SIGNATURES = {
                        _vp, _vp], ctypes.c_int),
     "dw_workspace_size": ([_vp, _i32, _i32], ctypes.c_size_t),
     "dw_model_set_chunk": ([_vp, _i32], ctypes.c_int),
+    "dw_model_set_workspace": ([_vp, _vp, ctypes.c_size_t], ctypes.c_int),
     "dw_model_set_tuning": ([_vp, ctypes.POINTER(Tuning)], ctypes.c_int),
     "dw_model_get_tuning": ([_vp, ctypes.POINTER(Tuning)], ctypes.c_int),
     "dw_model_conditioning": ([_vp, _vp, _i32, _vp], ctypes.c_int),
@@ -159,6 +160,16 @@ class NativeModel:
         t = Tuning()
         _check(self.lib.dw_model_get_tuning(self.handle, ctypes.byref(t)))
         return {f: int(getattr(t, f)) for f in TUNING_FIELDS}
+
+    def set_workspace(self, buf) -> None:
+        """Carve the activation buffers from `buf` (a CUDA uint8 tensor of >= workspace_bytes, kept alive by
+        the caller), or return to library-owned buffers with None."""
+        if buf is None:
+            _check(self.lib.dw_model_set_workspace(self.handle, None, 0))
+            self._ws = None
+            return
+        _check(self.lib.dw_model_set_workspace(self.handle, ctypes.c_void_p(buf.data_ptr()), buf.numel() * buf.element_size()))
+        self._ws = buf
 
     def set_chunk(self, walkers: int) -> None:
         _check(self.lib.dw_model_set_chunk(self.handle, int(walkers)))

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -35,8 +36,20 @@ dw_status fail(dw_status code, const std::string& msg) {
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool ext = false;  // points into a caller-owned workspace
+  void assign(void* ptr, size_t n) {
+    if (p && !ext) (void)hipFree(p);
+    p = ptr;
+    bytes = n;
+    ext = true;
+  }
   hipError_t ensure(size_t need) {
     if (need <= bytes) return hipSuccess;
+    if (ext) {  // leave the caller's region; fall back to an own allocation
+      p = nullptr;
+      bytes = 0;
+      ext = false;
+    }
     if (p) {
       hipError_t e = hipFree(p);
       if (e != hipSuccess) return e;
@@ -48,9 +61,10 @@ struct DevBuf {
     return e;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && !ext) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    ext = false;
   }
   template <typename T>
   T* as() const {
@@ -137,6 +151,8 @@ struct dw_model {
   bool dirty = true;
   int chunk = 0;
   dw_tuning tune{-1, -1, 0, -1, -1, 0};
+  void* ext_ws = nullptr;  // caller-owned workspace (dw_model_set_workspace)
+  size_t ext_ws_bytes = 0;
   GemmPrec prec = GEMM_F32;
   Weights wf, wd;    // float and double layouts
   Workspace sf, sd;  // float and double activations
@@ -291,7 +307,9 @@ dw_status prepare_weights(dw_model* m, Weights& W, bool dbl) {
     if ((st = upload(W.kappa, kf))) return st;
     if ((st = upload(W.wenv, wf))) return st;
   }
-  HIP_TRY(hipDeviceSynchronize());
+  // the uploads are hipMemcpy on the null stream, whose DMA may still be in flight on return: wait for the
+  // null stream only (a device-wide synchronisation would also wait for other models' streams)
+  HIP_TRY(hipStreamSynchronize(nullptr));
   W.ready = true;
   return DW_OK;
 }
@@ -343,6 +361,29 @@ dw_status ensure_ws(dw_model* m, Workspace& w, long B, int C, size_t esz) {
   const int Cp = rows_per_electron(C);
   const long rows = B * m->n * Cp;
   w.X.resize(m->L);
+  if (m->ext_ws) {  // carve the activation buffers out of the caller's workspace (256-byte aligned)
+    size_t off = 0;
+    bool fits = true;
+    auto carve = [&](DevBuf& b, size_t n) {
+      if (off + n > m->ext_ws_bytes) {
+        fits = false;
+        return;
+      }
+      b.assign((char*)m->ext_ws + off, n);
+      off += (n + 255) & ~(size_t)255;
+    };
+    int kmmax = 0;
+    for (int l = 0; l < m->L; ++l) kmmax = std::max(kmmax, m->Km[l]);
+    for (int l = 0; l < m->L; ++l) carve(w.X[l], (size_t)rows * m->Kx[l] * esz);
+    carve(w.MU, (size_t)B * Cp * kmmax * esz);
+    carve(w.V, (size_t)B * Cp * m->W1 * esz * (C == 1 ? 4 : 1));
+    if (esz == 8 && C > 1) carve(w.Z, (size_t)rows * m->W1 * esz);
+    carve(w.H, (size_t)rows * m->W1 * esz);
+    carve(w.O, (size_t)rows * m->NO * esz);
+    carve(w.A, (size_t)B * m->D * C * m->n * m->n * esz);
+    if (!fits)
+      return fail(DW_ERR_INVALID, "the workspace given to dw_model_set_workspace is smaller than dw_workspace_size");
+  }
   for (int l = 0; l < m->L; ++l) HIP_TRY(w.X[l].ensure((size_t)rows * m->Kx[l] * esz));
   int kmmax = 0;
   for (int l = 0; l < m->L; ++l) kmmax = std::max(kmmax, m->Km[l]);
@@ -645,6 +686,16 @@ dw_status dw_model_set_tuning(dw_model* m, const dw_tuning* t) {
 dw_status dw_model_get_tuning(const dw_model* m, dw_tuning* out) {
   if (!m || !out) return fail(DW_ERR_INVALID, "null argument");
   *out = m->tune;
+  return DW_OK;
+}
+
+dw_status dw_model_set_workspace(dw_model* m, void* device_ptr, size_t bytes) {
+  if (!m) return fail(DW_ERR_INVALID, "null model");
+  if (device_ptr && ((uintptr_t)device_ptr & 255)) return fail(DW_ERR_INVALID, "workspace must be 256-byte aligned");
+  m->sf.release();
+  m->sd.release();
+  m->ext_ws = device_ptr;
+  m->ext_ws_bytes = device_ptr ? bytes : 0;
   return DW_OK;
 }
 

@@ -1376,7 +1376,18 @@ __global__ __launch_bounds__(64) void k_det_value_lds(const TA* __restrict__ A, 
     __syncthreads();
     if (j > p && j < n) {
       const double up = tm[p * P + j];
-      for (int i = p + 1; i < n; ++i) tm[i * P + j] -= tm[i * P + p] * up;
+      int i = p + 1;
+      for (; i + 8 <= n; i += 8) {  // 8 rows per batch: all loads before the stores
+        double l8[8], a8[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          l8[q] = tm[(i + q) * P + p];
+          a8[q] = tm[(i + q) * P + j];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) tm[(i + q) * P + j] = a8[q] - l8[q] * up;
+      }
+      for (; i < n; ++i) tm[i * P + j] -= tm[i * P + p] * up;
     }
     __syncthreads();
   }

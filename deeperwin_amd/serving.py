@@ -9,9 +9,10 @@ electron count (2..n_el_max) and ion count run under the same weights.
 Scheduling (continuous batching at MCMC-slice granularity):
 * requests are admitted in arrival order while the sum of their device footprints (walker state plus
   the model workspace, ``dw_workspace_size``) stays under the HBM budget and a slot is free;
-* every admitted request owns a HIP stream, a native model instance (its own workspace) and its walker
-  state on the device; the host loop enqueues the next slice of MCMC steps of every active request
-  round-robin, so slices of different requests overlap on the device;
+* every admitted request owns a slot: a HIP stream, the slot's grow-only workspace arena
+  (``dw_model_set_workspace``: no device allocation per request once the slot has seen a request that
+  large), a native model instance from the shape pool and its walker state on the device; the host loop enqueues the next slice of
+  MCMC steps of every active request round-robin, so slices of different requests overlap on the device;
 * after the last slice the request's local energies and their FP64 mean are enqueued on its stream;
   completion is detected with a stream event, the slot and its memory budget are released and the next
   waiting request is admitted.
@@ -22,6 +23,7 @@ from __future__ import annotations
 
 import math
 import time
+from collections import OrderedDict
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Tuple
 
@@ -83,17 +85,22 @@ def poisson_stream(n_requests: int, rate: float = 50.0, seed: int = 5, n_el_rang
 
 
 class SharedWeights:
-    """One max-size parameter set; every request gets a native model built from its truncated slice.
+    """One max-size parameter set; every request runs a native model built from its truncated slice.
 
-    A model (and its workspace) lives exactly as long as its request: it is created at admission and
-    dropped when the request retires, so HBM use follows the admitted set."""
+    The per-geometry slice depends only on the shape (n_el, n_up, n_ion), so models are kept in a pool per
+    shape and handed out again: their weights stay resident in HBM, and a request whose shape has been
+    seen before costs no model creation, weight upload or device allocation.  Idle models beyond
+    `max_models` are destroyed, oldest first."""
 
     def __init__(self, n_el_max: int = 60, n_ion_max: int = 60, n_det: int = 8, seed: int = 0,
-                 precision: str = "fp32", chunk: int = 128):
+                 precision: str = "fp32", chunk: int = 128, max_models: int = 64):
         self.n_el_max, self.n_ion_max, self.n_det = n_el_max, n_ion_max, n_det
         self.big = init_params(ModelShape(n_el_max, (n_el_max + 1) // 2, n_ion_max, n_det), seed)
         self.precision = precision
         self.chunk = chunk
+        self.max_models = max_models
+        self._idle: "OrderedDict[tuple, List[object]]" = OrderedDict()
+        self._n_idle = 0
 
     def shape_for(self, req: Request) -> ModelShape:
         return ModelShape(req.n_el, req.n_up, len(req.Z), self.n_det)
@@ -104,19 +111,53 @@ class SharedWeights:
     def acquire(self, req: Request):
         from ._native import NativeModel
 
+        key = (req.n_el, req.n_up, len(req.Z))
+        if self._idle.get(key):
+            self._n_idle -= 1
+            m = self._idle[key].pop()
+            if not self._idle[key]:
+                del self._idle[key]
+            return m
         m = NativeModel(self.shape_for(req), self.precision, chunk=self.chunk)
         m.set_params(self.params_for(req))
+        m.pool_key = key
         return m
 
     def release(self, model) -> None:
-        del model  # the handle (weights + workspace) is destroyed with its last reference
+        """Return a model to the pool (its caller must have finished using it)."""
+        self._idle.setdefault(model.pool_key, []).append(model)
+        self._idle.move_to_end(model.pool_key)
+        self._n_idle += 1
+        while self._n_idle > self.max_models:
+            key, lst = next(iter(self._idle.items()))
+            lst.pop(0)  # destroyed with its last reference
+            self._n_idle -= 1
+            if not lst:
+                del self._idle[key]
+
+    def arena(self, slot: int, nbytes: int):
+        """Workspace buffer of a request slot: grow-only and kept across requests and servers, so device
+        memory is allocated only when a slot first meets a larger request."""
+        import torch
+
+        if not hasattr(self, "_arenas"):
+            self._arenas = {}
+        buf = self._arenas.get(slot)
+        if buf is None or buf.numel() < nbytes:
+            self._arenas[slot] = None
+            del buf
+            buf = torch.empty(max(nbytes, 1 << 20), dtype=torch.uint8, device="cuda")
+            self._arenas[slot] = buf
+        return buf[:nbytes]
+
+    def workspace_bytes(self, req: Request, model) -> int:
+        """The larger of the value-pass and Laplacian-pass workspaces (the latter per chunk of walkers)."""
+        return int(max(model.workspace_bytes(req.n_walkers, 0), model.workspace_bytes(req.n_walkers, 1)))
 
     def footprint(self, req: Request, model) -> int:
-        """Device bytes a request holds while active: walker state, the draws of one slice and the larger of
-        the value-pass and Laplacian-pass workspaces."""
-        ws = max(model.workspace_bytes(req.n_walkers, 0), model.workspace_bytes(req.n_walkers, 1))
+        """Device bytes a request holds while active: its workspace, walker state and the draws of a slice."""
         state = req.n_walkers * (req.n_el * 3 * 4 * 3 + 16)
-        return int(ws + state)
+        return int(self.workspace_bytes(req, model) + state)
 
 
 class _Active:
@@ -172,6 +213,10 @@ class _Active:
                 mean = torch.where(ok, ed, torch.zeros_like(ed)).sum() / cnt
                 var = torch.where(ok, (ed - mean) ** 2, torch.zeros_like(ed)).sum() / cnt
                 self.moments = torch.stack([mean, var, self.acc[:max(1, self.req.n_steps)].double().mean()])
+                # result to pinned host memory on the request's own stream: reading it after the event
+                # needs no further synchronisation
+                self.host = torch.empty(3, dtype=torch.float64, pin_memory=True)
+                self.host.copy_(self.moments, non_blocking=True)
                 self.done_event = torch.cuda.Event()
                 self.done_event.record(self.stream)
 
@@ -214,6 +259,8 @@ class Server:
                     break
                 pending.pop(0)
                 si = free_streams.pop(0)
+                # the slot's workspace arena: no device allocation once the slot has seen a request this big
+                model.set_workspace(self.w.arena(si, self.w.workspace_bytes(req, model)))
                 active.append((_Active(req, model, streams[si], now(), self.slice_steps), si, need))
                 used += need
             # one slice for every active request that still has steps to enqueue
@@ -224,10 +271,12 @@ class Server:
             still = []
             for st, si, need in active:
                 if st.done_event is not None and st.done_event.query():
-                    m = st.moments.cpu().numpy()
+                    m = st.host.numpy()
                     results.append(Result(st.req.rid, float(m[0]), float(m[1]), float(m[2]), st.req.arrival,
                                           st.admitted, now()))
+                    st.model.set_workspace(None)  # the request's work is done: its workspace returns to the cache
                     self.w.release(st.model)
+                    st.model = None
                     free_streams.append(si)
                     used -= need
                 else:

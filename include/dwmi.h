@@ -114,6 +114,15 @@ dw_status dw_model_conditioning(dw_model* m, float* out, int32_t n_walkers, dw_s
  * (0 = log|psi| / MCMC, 1 = local energy). */
 size_t dw_workspace_size(const dw_model* m, int32_t n_walkers, int32_t mode);
 
+/* Caller-owned workspace (no counterpart in the reference; the analogue of an XLA FFI scratch buffer).
+ * When set, the large activation buffers of every pass are carved from [device_ptr, device_ptr + bytes)
+ * instead of being allocated by the library, so a caller that recycles buffers (a serving loop, a
+ * graph) avoids device allocation and release per model.  bytes must be >= dw_workspace_size for every
+ * (n_walkers, mode) the model is then called with, else the call fails; the pointer must be 256-byte
+ * aligned and stay valid while the model may use it.  Calls on one model must not overlap (one stream).
+ * device_ptr = NULL returns to library-owned workspace. */
+dw_status dw_model_set_workspace(dw_model* m, void* device_ptr, size_t bytes);
+
 /* Cap on walkers processed per internal pass of dw_local_energy (0 = no cap).  Bounds the workspace. */
 dw_status dw_model_set_chunk(dw_model* m, int32_t max_walkers_per_pass);
 
