@@ -35,12 +35,12 @@ class ModelDesc(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("fwd_main_tile", ctypes.c_int32), ("fwd_orb_tile", ctypes.c_int32),
                 ("fwd_mu_split", ctypes.c_int32), ("lap_orb_tile", ctypes.c_int32),
-                ("lap_mu_tile", ctypes.c_int32), ("lap_electrons", ctypes.c_int32)]
+                ("lap_mu_tile", ctypes.c_int32), ("lap_electrons", ctypes.c_int32), ("lap_cols", ctypes.c_int32)]
 
 
 TUNING_FIELDS = [f for f, _ in Tuning._fields_]
 TUNING_DEFAULT = {"fwd_main_tile": -1, "fwd_orb_tile": -1, "fwd_mu_split": 0, "lap_orb_tile": -1,
-                  "lap_mu_tile": -1, "lap_electrons": 0}
+                  "lap_mu_tile": -1, "lap_electrons": 0, "lap_cols": 0}
 
 
 SIGNATURES = {

@@ -150,7 +150,7 @@ struct dw_model {
   std::vector<std::pair<std::string, std::vector<int64_t>>> expected;
   bool dirty = true;
   int chunk = 0;
-  dw_tuning tune{-1, -1, 0, -1, -1, 0};
+  dw_tuning tune{-1, -1, 0, -1, -1, 0, 0};
   void* ext_ws = nullptr;  // caller-owned workspace (dw_model_set_workspace)
   size_t ext_ws_bytes = 0;
   GemmPrec prec = GEMM_F32;
@@ -512,7 +512,7 @@ dw_status run_pass(dw_model* m, const float* r, long B, const float* R, const fl
       } else {
         gemm_lap(m->prec, X[l], m->Kx[l], W.Wmain[l].as<float>(), W.Wmain_hi[l].as<uint16_t>(),
                  W.Wmain_lo[l].as<uint16_t>(), m->Kx[l], ws.V.as<float>(), m->W1, outp, ldo, last ? 1 : 0, sh,
-                 m->tune.lap_electrons, s);
+                 m->tune.lap_electrons, m->tune.lap_cols, s);
       }
     }
   }
@@ -679,6 +679,8 @@ dw_status dw_model_set_tuning(dw_model* m, const dw_tuning* t) {
   const int Cp = rows_per_electron(3 * m->n + 2);
   if (t->lap_electrons < 0 || t->lap_electrons * Cp > 192)
     return fail(DW_ERR_INVALID, "lap_electrons * pad16(3 n + 2) must be <= 192");
+  if (!(t->lap_cols == 0 || t->lap_cols == 128 || t->lap_cols == 256))
+    return fail(DW_ERR_INVALID, "lap_cols must be 0, 128 or 256");
   m->tune = *t;
   return DW_OK;
 }

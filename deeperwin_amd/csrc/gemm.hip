@@ -452,14 +452,16 @@ namespace dwmi {
 //   y_0 = tanh z_0,  y_k = t' z_k (k = 1..3n),  y_lap = t' z_lap + t'' sum_k z_k^2,  padded channels 0.
 // MFMA: v_mfma_f32_16x16x4_f32 (FP32) or v_mfma_f32_16x16x32_bf16 (BF16 forms, A split hi/lo in LDS).
 // ------------------------------------------------------------------------------------------------
-template <int RB, int PREC>
+template <int RB, int PREC, int BN>
 __global__ __launch_bounds__(256) void k_gemm_lap(const float* __restrict__ X, int ldx,
                                                   const float* __restrict__ Bt, const uint16_t* __restrict__ Bhi,
                                                   const uint16_t* __restrict__ Blo, int K,
                                                   const float* __restrict__ V, int ldv, float* __restrict__ out,
                                                   int ldo, int permute, int E, Shape sh, int tilesN) {
+  static_assert(BN == 128 || (BN == 256 && PREC == 0), "256-column tiles exist for the FP32 form only");
   constexpr int ROWS = RB * 16;
-  constexpr int BN = 128;
+  constexpr int WN = BN / 4;    // columns per wave
+  constexpr int NQ = WN / 16;   // 16-column MFMA blocks per wave
   constexpr int BK = (PREC == 0) ? 16 : 32;  // FP32: 16 measured faster than 32 (2.05 vs 2.10 ms at N2)
   constexpr int LDK = BK + 8;  // bf16 row pitch (elements)
   // epilogue window: all 128 columns at once when the tile fits in 60 KB of LDS, else two passes of 64
@@ -481,11 +483,11 @@ __global__ __launch_bounds__(256) void k_gemm_lap(const float* __restrict__ X, i
   const int col0 = tile_n * BN;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 
-  v4f acc[RB][2];
+  v4f acc[RB][NQ];
 #pragma unroll
   for (int a = 0; a < RB; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < NQ; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
 
   if constexpr (PREC == 0) {
     float(*As)[ROWS + 4] = (float(*)[ROWS + 4])smem;
@@ -542,13 +544,15 @@ __global__ __launch_bounds__(256) void k_gemm_lap(const float* __restrict__ X, i
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
         const int kr = kk + (lane >> 4);
-        const float b0 = Bs[kr][wv * 32 + (lane & 15)];
-        const float b1 = Bs[kr][wv * 32 + 16 + (lane & 15)];
+        float bq[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) bq[q] = Bs[kr][wv * WN + q * 16 + (lane & 15)];
 #pragma unroll
         for (int rb2 = 0; rb2 < RB; ++rb2) {
           const float a = As[kr][rb2 * 16 + (lane & 15)];
-          acc[rb2][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, acc[rb2][0], 0, 0, 0);
-          acc[rb2][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, acc[rb2][1], 0, 0, 0);
+#pragma unroll
+          for (int q = 0; q < NQ; ++q)
+            acc[rb2][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bq[q], acc[rb2][q], 0, 0, 0);
         }
       }
     }
@@ -650,14 +654,14 @@ __global__ __launch_bounds__(256) void k_gemm_lap(const float* __restrict__ X, i
   const int nt = C - 2;
   const int chunk = (nt + G - 1) / G;
 #pragma unroll
-  for (int pass = 0; pass < 128 / NC; ++pass) {
+  for (int pass = 0; pass < BN / NC; ++pass) {
     __syncthreads();
-    const int wc0 = wv * 32 - pass * NC;  // this wave's first column inside the pass window
+    const int wc0 = wv * WN - pass * NC;  // this wave's first column inside the pass window
     if (wc0 >= 0 && wc0 < NC) {
 #pragma unroll
       for (int rb2 = 0; rb2 < RB; ++rb2)
 #pragma unroll
-        for (int q = 0; q < 2; ++q)
+        for (int q = 0; q < NQ; ++q)
 #pragma unroll
           for (int j = 0; j < 4; ++j) Es[rb2 * 16 + 4 * (lane >> 4) + j][wc0 + q * 16 + (lane & 15)] = acc[rb2][q][j];
     }
@@ -723,14 +727,36 @@ __global__ __launch_bounds__(256) void k_gemm_lap(const float* __restrict__ X, i
 template <int PREC>
 static void gemm_lap_rb(int RB, const float* X, int ldx, const float* Bt, const uint16_t* Bhi, const uint16_t* Blo,
                         int K, const float* V, int ldv, float* out, int ldo, int permute, int E, const Shape& sh,
-                        hipStream_t s) {
-  const int tilesN = sh.W1 / 128;
+                        int cols, hipStream_t s) {
   const long tilesM = ((long)sh.B * sh.n + E - 1) / E;
+  if constexpr (PREC == 0) {
+    // 256-column tiles (every output column of a row tile in one workgroup, so its A tile is read once)
+    if (cols == 256 && RB <= 6 && sh.W1 % 256 == 0) {
+      const int tilesN = sh.W1 / 256;
+      const dim3 grid((unsigned)(tilesM * tilesN));
+#define DW_LAP_CASE256(R)                                                                                            \
+  case R:                                                                                                            \
+    hipLaunchKernelGGL((k_gemm_lap<R, 0, 256>), grid, dim3(256), 0, s, X, ldx, Bt, Bhi, Blo, K, V, ldv, out, ldo, \
+                       permute, E, sh, tilesN);                                                                      \
+    return;
+      switch (RB) {
+        DW_LAP_CASE256(1)
+        DW_LAP_CASE256(2)
+        DW_LAP_CASE256(3)
+        DW_LAP_CASE256(4)
+        DW_LAP_CASE256(5)
+        DW_LAP_CASE256(6)
+        default: break;
+      }
+#undef DW_LAP_CASE256
+    }
+  }
+  const int tilesN = sh.W1 / 128;
   const dim3 grid((unsigned)(tilesM * tilesN));
-#define DW_LAP_CASE(R)                                                                                             \
-  case R:                                                                                                          \
-    hipLaunchKernelGGL((k_gemm_lap<R, PREC>), grid, dim3(256), 0, s, X, ldx, Bt, Bhi, Blo, K, V, ldv, out, ldo, \
-                       permute, E, sh, tilesN);                                                                    \
+#define DW_LAP_CASE(R)                                                                                                  \
+  case R:                                                                                                               \
+    hipLaunchKernelGGL((k_gemm_lap<R, PREC, 128>), grid, dim3(256), 0, s, X, ldx, Bt, Bhi, Blo, K, V, ldv, out, ldo, \
+                       permute, E, sh, tilesN);                                                                         \
     break;
   switch (RB) {
     DW_LAP_CASE(1)
@@ -756,16 +782,16 @@ int lap_electrons_per_tile(int Cp) {
 }
 
 void gemm_lap(GemmPrec prec, const float* X, int ldx, const float* Bt, const uint16_t* Bhi, const uint16_t* Blo, int K,
-              const float* V, int ldv, float* out, int ldo, int permute, const Shape& sh, int E_override,
+              const float* V, int ldv, float* out, int ldo, int permute, const Shape& sh, int E_override, int cols,
               hipStream_t s) {
   const int E = E_override > 0 ? E_override : lap_electrons_per_tile(sh.Cp);
   const int RB = E * sh.Cp / 16;
   if (prec == GEMM_F32)
-    gemm_lap_rb<0>(RB, X, ldx, Bt, Bhi, Blo, K, V, ldv, out, ldo, permute, E, sh, s);
+    gemm_lap_rb<0>(RB, X, ldx, Bt, Bhi, Blo, K, V, ldv, out, ldo, permute, E, sh, cols, s);
   else if (prec == GEMM_BF16X3)
-    gemm_lap_rb<1>(RB, X, ldx, Bt, Bhi, Blo, K, V, ldv, out, ldo, permute, E, sh, s);
+    gemm_lap_rb<1>(RB, X, ldx, Bt, Bhi, Blo, K, V, ldv, out, ldo, permute, E, sh, cols, s);
   else
-    gemm_lap_rb<2>(RB, X, ldx, Bt, Bhi, Blo, K, V, ldv, out, ldo, permute, E, sh, s);
+    gemm_lap_rb<2>(RB, X, ldx, Bt, Bhi, Blo, K, V, ldv, out, ldo, permute, E, sh, cols, s);
 }
 
 }  // namespace dwmi

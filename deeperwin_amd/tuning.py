@@ -2,7 +2,8 @@
 
 The reference leaves tiling to XLA.  Here every model exposes its launch-shape choices through
 ``dw_model_set_tuning`` (FP32 GEMM tiles of the value and Laplacian passes, the K-split of the per-walker
-spin-mean GEMM of the value pass, electrons per workgroup of the fused Laplacian GEMM).  The tuner times
+spin-mean GEMM of the value pass, electrons and output columns per workgroup of the fused Laplacian
+GEMM).  The tuner times
 the pass each choice affects on real inputs of one representative geometry per bucket and keeps the
 fastest value of each field in turn (one coordinate-descent sweep); the table then applies to every model
 whose electron count falls in that bucket.  Choices change launch shapes only: every candidate computes
@@ -46,6 +47,7 @@ def search_space(n_el: int) -> List[Tuple[str, Sequence[int], str]]:
         ("fwd_main_tile", TILE_CHOICES, "value"),
         ("fwd_mu_split", SPLIT_CHOICES, "value"),
         ("fwd_orb_tile", TILE_CHOICES, "value"),
+        ("lap_cols", (128, 256), "lap"),
         ("lap_electrons", lap_electron_choices(n_el), "lap"),
         ("lap_mu_tile", TILE_CHOICES, "lap"),
         ("lap_orb_tile", TILE_CHOICES, "lap"),

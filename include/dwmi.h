@@ -136,6 +136,8 @@ typedef struct dw_tuning {
   int32_t lap_orb_tile;   /* orbital GEMMs of the Laplacian pass                                     */
   int32_t lap_mu_tile;    /* spin-mean GEMMs of the Laplacian pass                                   */
   int32_t lap_electrons;  /* electrons per workgroup of the fused Laplacian GEMM (E * Cp <= 192)      */
+  int32_t lap_cols;       /* output columns per workgroup of that GEMM: 0 / 128, or 256 (FP32 form,
+                             E * Cp <= 96; larger tiles fall back to 128)                             */
 } dw_tuning;
 dw_status dw_model_set_tuning(dw_model* m, const dw_tuning* t);
 dw_status dw_model_get_tuning(const dw_model* m, dw_tuning* out);

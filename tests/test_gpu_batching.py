@@ -134,3 +134,49 @@ def test_tuning_changes_launch_shapes_only(cuda):
     lp3, e3 = run()
     assert np.all(np.abs(lp3 - lp0) <= tol_lp)
     assert np.all(np.abs(e3 - e0) <= tol_e)
+
+
+def test_caller_workspace_gives_identical_results(cuda):
+    """dw_model_set_workspace: the same passes with activation buffers carved from a caller buffer."""
+    from deeperwin_amd._native import DwError, NativeModel
+    from deeperwin_amd.params import init_params
+    from deeperwin_amd.physics import generate_el_ion_mapping, init_walkers_gaussian
+
+    R, Z = _geoms(k=1, seed=14, n_max=12)[0]
+    n = int(Z.sum())
+    shape = ModelShape(n, (n + 1) // 2, len(Z), 4)
+    params = init_params(shape, 3)
+    B = 96
+    r0 = init_walkers_gaussian(np.random.default_rng(2), B, R, generate_el_ion_mapping(R, Z, n, shape.n_up))
+    r = torch.as_tensor(r0, dtype=torch.float32, device="cuda").contiguous()
+    Rd = torch.as_tensor(R, dtype=torch.float32, device="cuda")
+    Zd = torch.as_tensor(Z, dtype=torch.float32, device="cuda")
+
+    def run(m):
+        lp = torch.empty(B, device="cuda")
+        e = torch.empty(B, device="cuda")
+        m.logpsi(r, Rd, Zd, lp)
+        m.local_energy(r, Rd, Zd, e)
+        torch.cuda.synchronize()
+        return lp.cpu().numpy(), e.cpu().numpy()
+
+    own = NativeModel(shape, chunk=32)
+    own.set_params(params)
+    lp0, e0 = run(own)
+    ext = NativeModel(shape, chunk=32)
+    ext.set_params(params)
+    need = max(ext.workspace_bytes(B, 0), ext.workspace_bytes(B, 1))
+    buf = torch.full((need,), 255, dtype=torch.uint8, device="cuda")  # non-zero garbage to start from
+    ext.set_workspace(buf)
+    for _ in range(2):
+        lp1, e1 = run(ext)
+        np.testing.assert_array_equal(lp1, lp0)
+        np.testing.assert_array_equal(e1, e0)
+    small = NativeModel(shape)
+    small.set_params(params)
+    small.set_workspace(torch.empty(4096, dtype=torch.uint8, device="cuda"))
+    with pytest.raises(DwError):
+        small.logpsi(r, Rd, Zd, torch.empty(B, device="cuda"))
+    ext.set_workspace(None)
+    lp2, e2 = run(ext)
+    np.testing.assert_array_equal(e2, e0)
