@@ -54,7 +54,9 @@ def parse():
     ap.add_argument("--requests", type=int, default=24, help="serving: requests in the stream")
     ap.add_argument("--rate", type=float, default=50.0, help="serving: offered load, requests/s")
     ap.add_argument("--fp64-threshold", type=float, default=1e6)
-    ap.add_argument("--no-tune", action="store_true", help="keep the built-in launch-shape heuristics")
+    ap.add_argument("--tune", action="store_true",
+                    help="n2: pick launch shapes with the autotuner first (default: the built-in heuristics, so "
+                         "every run times the same kernels)")
     ap.add_argument("--burn-in", type=int, default=300, help="c4h2-precision: MCMC steps before the second study")
     return ap.parse_args()
 
@@ -397,7 +399,7 @@ def main():
     gen.manual_seed(1)
 
     tuning = None
-    if not args.no_tune:  # launch shapes from the autotuner, on the walkers of this run
+    if args.tune:  # launch shapes from the autotuner, on the walkers of this run
         from deeperwin_amd.tuning import autotune_model
 
         tuning, _ = autotune_model(model, r, Rd, Zd, reps=3)

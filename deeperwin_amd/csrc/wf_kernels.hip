@@ -526,21 +526,20 @@ static void twoel_mfma_dispatch(const float* r, const TwoElOut<float>& o, const 
 template <bool LAP, typename T>
 static void twoel_dispatch(const float* r, const TwoElOut<T>& o, const T* W2e, const T* b2e, const Shape& sh,
                            hipStream_t s) {
+  // FP32: MFMA kernel (width2 == 32 is enforced at model creation); FP64 form: the VALU kernel
   if constexpr (sizeof(T) == 4) {
-    if (sh.W2 == 32) {
-      switch (sh.L) {
-        case 2: twoel_mfma_dispatch<LAP, 2>(r, o, W2e, b2e, sh, s); break;
-        case 3: twoel_mfma_dispatch<LAP, 3>(r, o, W2e, b2e, sh, s); break;
-        default: twoel_mfma_dispatch<LAP, 4>(r, o, W2e, b2e, sh, s); break;
-      }
-      return;
+    switch (sh.L) {
+      case 2: twoel_mfma_dispatch<LAP, 2>(r, o, W2e, b2e, sh, s); break;
+      case 3: twoel_mfma_dispatch<LAP, 3>(r, o, W2e, b2e, sh, s); break;
+      default: twoel_mfma_dispatch<LAP, 4>(r, o, W2e, b2e, sh, s); break;
     }
-  }
-  const unsigned grid = (unsigned)(sh.B * ((sh.n + 7) / 8));
-  switch (sh.L) {
-    case 2: hipLaunchKernelGGL((k_twoel<LAP, 2, T>), dim3(grid), dim3(256), 0, s, r, o, W2e, b2e, sh); break;
-    case 3: hipLaunchKernelGGL((k_twoel<LAP, 3, T>), dim3(grid), dim3(256), 0, s, r, o, W2e, b2e, sh); break;
-    default: hipLaunchKernelGGL((k_twoel<LAP, 4, T>), dim3(grid), dim3(256), 0, s, r, o, W2e, b2e, sh); break;
+  } else {
+    const unsigned grid = (unsigned)(sh.B * ((sh.n + 7) / 8));
+    switch (sh.L) {
+      case 2: hipLaunchKernelGGL((k_twoel<LAP, 2, T>), dim3(grid), dim3(256), 0, s, r, o, W2e, b2e, sh); break;
+      case 3: hipLaunchKernelGGL((k_twoel<LAP, 3, T>), dim3(grid), dim3(256), 0, s, r, o, W2e, b2e, sh); break;
+      default: hipLaunchKernelGGL((k_twoel<LAP, 4, T>), dim3(grid), dim3(256), 0, s, r, o, W2e, b2e, sh); break;
+    }
   }
 }
 
@@ -1095,11 +1094,10 @@ __global__ __launch_bounds__(64 * WPB) void k_det_lap(const TA* __restrict__ A, 
   }
 }
 
-// Large matrices (n > 32): one 256-thread workgroup per matrix, FP64 in LDS.  Element-parallel
-// partial-pivot LU (wave 0 searches the pivot), inverse by per-column substitution, and for every tangent
-// channel the pair (a, c) / (c, a) of Y = M^-1 dM_k is formed by one thread, so tr Y and tr Y^2 need no
-// stored Y.
-template <typename TA, bool LAP>
+// Laplacian mode, large matrices (n > 32): one 256-thread workgroup per matrix, FP64 in LDS.
+// Element-parallel partial-pivot LU (wave 0 searches the pivot), inverse by per-column substitution, then
+// the tangent channels on FP64 MFMA blocks (below).
+template <typename TA>
 __global__ __launch_bounds__(256) void k_det_lds(const TA* __restrict__ A, double* __restrict__ ld,
                                                  double* __restrict__ sd, double* __restrict__ g,
                                                  double* __restrict__ lapd, float* __restrict__ dcond, int n, int C) {
@@ -1120,7 +1118,7 @@ __global__ __launch_bounds__(256) void k_det_lds(const TA* __restrict__ A, doubl
   if (tid < n) perm[tid] = tid;
   if (tid == 0) sing_sh = 0;
   __syncthreads();
-  if (LAP) {  // 1-norm of M (max column sum)
+  {  // 1-norm of M (max column sum)
     for (int c = tid; c < n; c += 256) {
       double cs = 0.0;
       for (int rr = 0; rr < n; ++rr) cs += fabs(T[rr * P + c]);
@@ -1212,14 +1210,6 @@ __global__ __launch_bounds__(256) void k_det_lds(const TA* __restrict__ A, doubl
   const bool sing = sing_sh != 0;
   const double logabs = sing ? -INFINITY : lsum;
   const double sgn = sing ? 0.0 : ((((int)negs + swaps) & 1) ? -1.0 : 1.0);
-  if (!LAP) {
-    if (tid == 0) {
-      ld[bd] = logabs;
-      sd[bd] = sgn;
-      dcond[bd] = (float)(gmax / gmin);
-    }
-    return;
-  }
   __syncthreads();
   // inverse: column j solves L U x = P e_j
   for (int j = tid; j < n; j += 256) {
@@ -1410,9 +1400,9 @@ void launch_det(const TA* A, double* ld, double* sd, double* g, double* lapd, fl
       const size_t shv = (size_t)n * (n + 1) * sizeof(double);  // <= 31.5 KB for n <= 63
       hipLaunchKernelGGL((k_det_value_lds<TA>), dim3((unsigned)nmat), dim3(64), shv, s, A, ld, sd, dcond, n, stride);
     } else {
-      if (shm > 65536) (void)hipFuncSetAttribute((const void*)k_det_lds<TA, true>,
+      if (shm > 65536) (void)hipFuncSetAttribute((const void*)k_det_lds<TA>,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-      hipLaunchKernelGGL((k_det_lds<TA, true>), dim3((unsigned)nmat), dim3(256), shm, s, A, ld, sd, g, lapd, dcond, n,
+      hipLaunchKernelGGL((k_det_lds<TA>), dim3((unsigned)nmat), dim3(256), shm, s, A, ld, sd, g, lapd, dcond, n,
                          sh.C);
     }
     return;
@@ -1456,8 +1446,7 @@ template void launch_det<double>(const double*, double*, double*, double*, doubl
 // E_kin = -1/2 (Lap f + |grad f|^2)  (hamiltonian.py:216, 266 with log psi^2 = 2 f); E_pot as
 // hamiltonian.py:17-39.
 // ------------------------------------------------------------------------------------------------
-template <bool LAP>
-__global__ void k_combine(const double* __restrict__ ld, const double* __restrict__ sd, const double* __restrict__ g,
+__global__ void k_combine_value(const double* __restrict__ ld, const double* __restrict__ sd, const double* __restrict__ g,
                           const double* __restrict__ lapd, const float* __restrict__ dcond, float cond_thr,
                           const float* __restrict__ r, const float* __restrict__ R, const float* __restrict__ Z,
                           float* __restrict__ out_lp, float* __restrict__ out_phase, float* __restrict__ out_eloc,
@@ -1490,66 +1479,11 @@ __global__ void k_combine(const double* __restrict__ ld, const double* __restric
   if (out_lp) out_lp[b] = lp;
   if (out_phase) out_phase[b] = (S < 0.0) ? 3.14159265358979323846f : 0.f;
   if (!isfinite(lp)) fl |= DWF_NONFINITE;
-  if (LAP) {
-    const double sgnS = (S > 0.0) ? 1.0 : ((S < 0.0) ? -1.0 : 0.0);
-    const double den = aS + eps;
-    const double* gb = g + b * D * nd;
-    double lapm = 0.0;
-    for (int d = 0; d < D; ++d)
-      if ((double)l[d] == m) lapm += lapd[b * D + d];
-    lapm /= cnt;
-    double sum_d2S = 0.0, sum_dS2 = 0.0, sum_df2 = 0.0;
-    for (int k = 0; k < nd; ++k) {
-      double dm = 0.0;
-      for (int d = 0; d < D; ++d)
-        if ((double)l[d] == m) dm += gb[d * nd + k];
-      dm /= cnt;
-      double dS = 0.0, d2 = 0.0;
-      for (int d = 0; d < D; ++d) {
-        const double e = (double)sg[d] * exp((double)l[d] - m);
-        const double x = (double)gb[d * nd + k] - dm;
-        dS += e * x;
-        d2 += e * x * x;
-      }
-      sum_d2S += d2;
-      sum_dS2 += dS * dS;
-      const double df = sgnS * dS / den + dm;
-      sum_df2 += df * df;
-    }
-    for (int d = 0; d < D; ++d) {
-      const double e = (double)sg[d] * exp((double)l[d] - m);
-      sum_d2S += e * ((double)lapd[b * D + d] - lapm);
-    }
-    const double lapf = sgnS * sum_d2S / den - sum_dS2 / (den * den) + lapm;
-    const double ekin = -0.5 * (lapf + sum_df2);
-    // potential energy
-    const float* rb = r + b * n * 3;
-    double epot = 0.0;
-    for (int i = 0; i < n; ++i)
-      for (int j = i + 1; j < n; ++j) {
-        const double dx = rb[i * 3] - rb[j * 3], dy = rb[i * 3 + 1] - rb[j * 3 + 1], dz = rb[i * 3 + 2] - rb[j * 3 + 2];
-        epot += 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
-      }
-    for (int i = 0; i < n; ++i)
-      for (int I = 0; I < sh.I; ++I) {
-        const double dx = rb[i * 3] - R[I * 3], dy = rb[i * 3 + 1] - R[I * 3 + 1], dz = rb[i * 3 + 2] - R[I * 3 + 2];
-        epot -= (double)Z[I] / sqrt(dx * dx + dy * dy + dz * dz);
-      }
-    for (int I = 0; I < sh.I; ++I)
-      for (int J = I + 1; J < sh.I; ++J) {
-        const double dx = R[I * 3] - R[J * 3], dy = R[I * 3 + 1] - R[J * 3 + 1], dz = R[I * 3 + 2] - R[J * 3 + 2];
-        epot += (double)Z[I] * Z[J] / sqrt(dx * dx + dy * dy + dz * dz);
-      }
-    const float eloc = (float)(ekin + epot);
-    if (!isfinite(eloc)) fl |= DWF_NONFINITE;
-    if (out_eloc) out_eloc[b] = eloc;
-  }
   if (out_flags) out_flags[b] = fl;
 }
 
-// Laplacian pass: one wavefront per walker.  Lanes split the 3n gradient components and the electrons
-// of the potential; partial sums meet in a fixed shuffle tree (deterministic).  Same formulas as
-// k_combine<true>.
+// Laplacian pass: one wavefront per walker; the formulas above.  Lanes split the 3n gradient components
+// and the electrons of the potential; partial sums meet in a fixed shuffle tree (deterministic).
 __global__ __launch_bounds__(256) void k_combine_lap(const double* __restrict__ ld, const double* __restrict__ sd,
                                                      const double* __restrict__ g, const double* __restrict__ lapd,
                                                      const float* __restrict__ dcond, float cond_thr,
@@ -1651,7 +1585,7 @@ void launch_combine(const double* ld, const double* sd, const double* g, const d
     hipLaunchKernelGGL(k_combine_lap, dim3((unsigned)((sh.B + 3) / 4)), dim3(256), 0, s, ld, sd, g, lapd, dcond,
                        cond_thr, r, R, Z, out_lp, out_phase, out_eloc, out_flags, out_kappa, sh);
   else
-    hipLaunchKernelGGL(k_combine<false>, dim3(grid), dim3(128), 0, s, ld, sd, g, lapd, dcond, cond_thr, r, R, Z,
+    hipLaunchKernelGGL(k_combine_value, dim3(grid), dim3(128), 0, s, ld, sd, g, lapd, dcond, cond_thr, r, R, Z,
                        out_lp, out_phase, out_eloc, out_flags, out_kappa, sh);
 }
 
