@@ -501,6 +501,13 @@ dw_status run_pass(dw_model* m, const float* r, long B, const float* R, const fl
       em.tile = m->tune.fwd_main_tile;
       em.act_tanh = 1;
       em.permute = last ? 1 : 0;
+      if constexpr (!DBL) {
+        if (m->prec == GEMM_F32 && m->tune.fwd_main_tile == 3) {  // 16x16-MFMA tiles of 96 electrons
+          gemm_value16(X[l], m->Kx[l], W.Wmain[l].as<float>(), m->Kx[l], ws.V.as<float>(), m->W1, vs, vstride, outp,
+                       ldo, last ? 1 : 0, sh, s);
+          continue;
+        }
+      }
       gemm_t(m, X[l], m->Kx[l], W.Wmain[l].as<T>(), W.Wmain_hi[l], W.Wmain_lo[l], m->Kx[l], outp, ldo, rows, m->W1,
              m->Kx[l], em, s);
     } else {
@@ -672,8 +679,9 @@ int32_t dw_model_ready(const dw_model* m) {
 
 dw_status dw_model_set_tuning(dw_model* m, const dw_tuning* t) {
   if (!m || !t) return fail(DW_ERR_INVALID, "null argument");
-  for (int32_t v : {t->fwd_main_tile, t->fwd_orb_tile, t->lap_orb_tile, t->lap_mu_tile})
+  for (int32_t v : {t->fwd_orb_tile, t->lap_orb_tile, t->lap_mu_tile})
     if (v < -1 || v > 2) return fail(DW_ERR_INVALID, "tile must be -1, 0, 1 or 2");
+  if (t->fwd_main_tile < -1 || t->fwd_main_tile > 3) return fail(DW_ERR_INVALID, "fwd_main_tile must be -1 .. 3");
   if (!(t->fwd_mu_split == 0 || t->fwd_mu_split == 1 || t->fwd_mu_split == 2 || t->fwd_mu_split == 4))
     return fail(DW_ERR_INVALID, "fwd_mu_split must be 0 (auto), 1, 2 or 4");
   const int Cp = rows_per_electron(3 * m->n + 2);

@@ -452,13 +452,15 @@ namespace dwmi {
 //   y_0 = tanh z_0,  y_k = t' z_k (k = 1..3n),  y_lap = t' z_lap + t'' sum_k z_k^2,  padded channels 0.
 // MFMA: v_mfma_f32_16x16x4_f32 (FP32) or v_mfma_f32_16x16x32_bf16 (BF16 forms, A split hi/lo in LDS).
 // ------------------------------------------------------------------------------------------------
-template <int RB, int PREC, int BN>
+template <int RB, int PREC, int BN, bool VAL = false>
 __global__ __launch_bounds__(256) void k_gemm_lap(const float* __restrict__ X, int ldx,
                                                   const float* __restrict__ Bt, const uint16_t* __restrict__ Bhi,
                                                   const uint16_t* __restrict__ Blo, int K,
                                                   const float* __restrict__ V, int ldv, float* __restrict__ out,
-                                                  int ldo, int permute, int E, Shape sh, int tilesN) {
+                                                  int ldo, int permute, int E, Shape sh, int tilesN, int vsplit = 1,
+                                                  long vstride = 0) {
   static_assert(BN == 128 || (BN == 256 && PREC == 0), "256-column tiles exist for the FP32 form only");
+  static_assert(!VAL || PREC == 0, "the value-pass variant is FP32");
   constexpr int ROWS = RB * 16;
   constexpr int WN = BN / 4;    // columns per wave
   constexpr int NQ = WN / 16;   // 16-column MFMA blocks per wave
@@ -642,6 +644,32 @@ __global__ __launch_bounds__(256) void k_gemm_lap(const float* __restrict__ X, i
       }
     }
   }
+  if constexpr (VAL) {
+    // value pass (one row per electron): y = tanh(acc + V[walker]) straight from the accumulators, V the
+    // sum of its K-split parts (bias included), rows optionally to spin-major order
+    const int dn = n - sh.u;
+#pragma unroll
+    for (int rb2 = 0; rb2 < RB; ++rb2)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const long r = row0 + rb2 * 16 + 4 * (lane >> 4) + j;
+          if (r < rows_total) {
+            const int c = col0 + wv * WN + q * 16 + (lane & 15);
+            const long b = div_idx(r, n);
+            const int i = (int)(r - b * n);
+            const float* Vp = V + b * (long)ldv + c;
+            float vs = Vp[0];
+            for (int t = 1; t < vsplit; ++t) vs += Vp[t * vstride];
+            const float y = tanh_fast(acc[rb2][q][j] + vs);
+            long orow = r;
+            if (permute) orow = (i < sh.u) ? b * sh.u + i : (long)sh.B * sh.u + b * dn + (i - sh.u);
+            out[orow * ldo + c] = y;
+          }
+        }
+    return;
+  }
   // ---- epilogue: tile to LDS, then (electron, column) pairs with lanes along columns, so every store of
   // a wave is one contiguous 256-byte row segment.  When a pass has fewer than 256 pairs, G threads share
   // a pair and split its tangent channels into contiguous chunks; their partial sum_k z_k^2 meet in LDS
@@ -774,6 +802,19 @@ static void gemm_lap_rb(int RB, const float* X, int ldx, const float* Bt, const 
     default: break;
   }
 #undef DW_LAP_CASE
+}
+
+void gemm_value16(const float* X, int ldx, const float* Bt, int K, const float* V, int ldv, int vsplit, long vstride,
+                  float* out, int ldo, int permute, const Shape& sh, hipStream_t s) {
+  // 96-electron x 128-column tiles on 16x16x4 MFMA (the Laplacian GEMM's main loop), one row per electron
+  Shape v = sh;
+  v.C = 1;
+  v.Cp = 1;
+  const int E = 96;
+  const int tilesN = sh.W1 / 128;
+  const long tilesM = ((long)sh.B * sh.n + E - 1) / E;
+  hipLaunchKernelGGL((k_gemm_lap<6, 0, 128, true>), dim3((unsigned)(tilesM * tilesN)), dim3(256), 0, s, X, ldx, Bt,
+                     nullptr, nullptr, K, V, ldv, out, ldo, permute, E, v, tilesN, vsplit, vstride);
 }
 
 int lap_electrons_per_tile(int Cp) {

@@ -51,6 +51,11 @@ void gemm_lap(GemmPrec prec, const float* X, int ldx, const float* Bt, const uin
               hipStream_t s);
 int lap_electrons_per_tile(int Cp);
 
+// Value-pass 1-el GEMM on the Laplacian GEMM's 16x16 MFMA main loop (FP32): out = tanh(X W + V[walker]),
+// V summed over vsplit parts vstride apart; permute sends rows to spin-major order.
+void gemm_value16(const float* X, int ldx, const float* Bt, int K, const float* V, int ldv, int vsplit, long vstride,
+                  float* out, int ldo, int permute, const Shape& sh, hipStream_t s);
+
 template <typename T>
 void launch_features0(const float* r, const float* R, T* X0, int Kx0, const Shape& sh, hipStream_t s);
 template <typename T>

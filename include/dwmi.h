@@ -130,7 +130,7 @@ dw_status dw_model_set_chunk(dw_model* m, int32_t max_walkers_per_pass);
  * the autotuner in deeperwin_amd/tuning.py per electron-count bucket; -1 / 0 keep the built-in heuristic.
  * FP32 GEMM tiles: 0 = 64x64, 1 = 64x128, 2 = 128x128 per 256-thread workgroup. */
 typedef struct dw_tuning {
-  int32_t fwd_main_tile;  /* 1-el embedding GEMMs of the value pass                                  */
+  int32_t fwd_main_tile;  /* 1-el embedding GEMMs of the value pass (3: 96 x 128 tiles, 16x16 MFMA)   */
   int32_t fwd_orb_tile;   /* orbital GEMMs of the value pass                                         */
   int32_t fwd_mu_split;   /* K-split (1, 2 or 4) of the spin-mean GEMMs of the value pass (FP32 form) */
   int32_t lap_orb_tile;   /* orbital GEMMs of the Laplacian pass                                     */
