@@ -39,6 +39,29 @@ __device__ __forceinline__ double wave_sum_d(double v) {
   return v;
 }
 
+// Wavefront sums of G = 1, 2 or 4 values at once.  The first log2(G) exchange steps (xor 32, then 16) trade
+// the values a lane is not responsible for, so every later step moves one value; lane l ends with the
+// total of v[l / (64 / G)].  Each total is paired in the same order as wave_sum_d (xor 32, 16, ..., 1).
+template <int G>
+__device__ __forceinline__ double wave_sum_group(const double (&v)[G], int lane) {
+  static_assert(G == 1 || G == 2 || G == 4, "groups of 1, 2 or 4");
+  double s;
+  if constexpr (G == 1) {
+    return wave_sum_d(v[0]);
+  } else if constexpr (G == 4) {
+    const bool h5 = (lane & 32) != 0, h4 = (lane & 16) != 0;
+    const double a = (h5 ? v[2] : v[0]) + __shfl_xor(h5 ? v[0] : v[2], 32, 64);
+    const double b = (h5 ? v[3] : v[1]) + __shfl_xor(h5 ? v[1] : v[3], 32, 64);
+    s = (h4 ? b : a) + __shfl_xor(h4 ? a : b, 16, 64);
+  } else {
+    const bool h5 = (lane & 32) != 0;
+    s = (h5 ? v[1] : v[0]) + __shfl_xor(h5 ? v[0] : v[1], 32, 64);
+  }
+#pragma unroll
+  for (int o = (G == 4 ? 8 : 16); o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  return s;
+}
+
 // Block sum for blockDim.x a multiple of 64 (<= 1024); `scratch` holds >= 16 floats.  All threads get
 // the result.
 __device__ __forceinline__ float block_sum(float v, float* scratch) {

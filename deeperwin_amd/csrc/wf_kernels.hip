@@ -995,85 +995,91 @@ __global__ __launch_bounds__(64 * WPB) void k_det_lap(const TA* __restrict__ A, 
   }
   __syncthreads();
 
+  // Channel passes on FP64 MFMA (v_mfma_f64_16x16x4f64) over NB x NB blocks of 16.  Lane l holds
+  // M^-1[I*16 + l%16][K*16 + l/16 + 4s] (A operand of Y_k = M^-1 dM_k) and dM_k[K*16 + l/16 + 4s][J*16 + l%16]
+  // (its B operand); the same registers are the B and A operands of (Y_k^T)_IJ = sum_K (dM_k)_KI^T (M^-1)_JK^T.
+  // Both products come out in one layout, so tr Y_k^2 is the sum of their elementwise product, and tr Y_k
+  // is the sum of the operand products.  Channels run in groups of G: the loads of the next group are
+  // issued before the current group is multiplied, and the G traces tr Y_k leave the wavefront in one
+  // grouped reduction (same pairing order as a single wave sum).
   double qacc = 0.0;
-  if constexpr (NMAX == 16) {
-    // Channel passes on FP64 MFMA (v_mfma_f64_16x16x4f64).  Lane l holds M^-1[l%16][l/16 + 4s] (A operand
-    // of Y_k = M^-1 dM_k) and dM_k[l/16 + 4s][l%16] (its B operand); the same registers are the B and A
-    // operands of Y_k^T = dM_k^T M^-T.  Both products come out in one layout, so tr Y_k^2 is the sum of
-    // their elementwise product, and tr Y_k is the sum of the operand products.
+  {
+    constexpr int NB = NMAX / 16, G = (NMAX == 16) ? 2 : 1;  // measured: 2 / 1 fastest at n = 14 / 26
     const int ri = lane & 15, kq = lane >> 4;
-    double miA[4];
+    double miA[NB][NB][4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) miA[q] = sMi[ri * NMAX + kq + 4 * q];
-    const bool jin = ri < n;
-    for (int k = 0; k < nd; ++k) {
-      const TA* Tg = Ab + (long)(1 + k) * nn;
-      double dmB[4];
-      double gp = 0.0;
+    for (int I = 0; I < NB; ++I)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int kk = kq + 4 * q;
-        dmB[q] = (jin && kk < n) ? (double)Tg[kk * n + ri] : 0.0;
-        gp += miA[q] * dmB[q];
-      }
-      v4d y = {0.0, 0.0, 0.0, 0.0}, yt = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        y = __builtin_amdgcn_mfma_f64_16x16x4f64(miA[q], dmB[q], y, 0, 0, 0);
-        yt = __builtin_amdgcn_mfma_f64_16x16x4f64(dmB[q], miA[q], yt, 0, 0, 0);
-      }
-      qacc += y[0] * yt[0] + y[1] * yt[1] + y[2] * yt[2] + y[3] * yt[3];
-      gp = wave_sum_d(gp);
-      if (lane == 0 && valid_mat) g[bd * nd + k] = gp;
-    }
-    qacc = wave_sum_d(qacc);
-  } else if constexpr (NMAX == 32) {
-    // The same on 2 x 2 blocks of 16: lane l holds M^-1[I*16 + l%16][K*16 + l/16 + 4s] and
-    // dM_k[K*16 + l/16 + 4s][J*16 + l%16] for all blocks; Y_IJ = sum_K M^-1_IK dM_KJ and
-    // (Y^T)_IJ = sum_K (dM_KI)^T (M^-1_JK)^T reuse the same registers.
-    const int ri = lane & 15, kq = lane >> 4;
-    double miA[2][2][4];
-#pragma unroll
-    for (int I = 0; I < 2; ++I)
-#pragma unroll
-      for (int K = 0; K < 2; ++K)
+      for (int K = 0; K < NB; ++K)
 #pragma unroll
         for (int q = 0; q < 4; ++q) miA[I][K][q] = sMi[(I * 16 + ri) * NMAX + K * 16 + kq + 4 * q];
-    for (int k = 0; k < nd; ++k) {
-      const TA* Tg = Ab + (long)(1 + k) * nn;
-      double dmB[2][2][4];
+    TA cur[G][NB][NB][4], nxt[G][NB][NB][4];
+    auto load = [&](TA (&d)[G][NB][NB][4], int k0) {
 #pragma unroll
-      for (int K = 0; K < 2; ++K)
+      for (int c = 0; c < G; ++c) {
+        const int k = k0 + c;
+        const TA* Tg = Ab + (long)(1 + (k < nd ? k : 0)) * nn;
 #pragma unroll
-        for (int J = 0; J < 2; ++J)
+        for (int K = 0; K < NB; ++K)
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int rr = K * 16 + kq + 4 * q, cc = J * 16 + ri;
-            dmB[K][J][q] = (rr < n && cc < n) ? (double)Tg[rr * n + cc] : 0.0;
-          }
-      double gp = 0.0;
-#pragma unroll
-      for (int I = 0; I < 2; ++I)
-#pragma unroll
-        for (int K = 0; K < 2; ++K)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) gp += miA[I][K][q] * dmB[K][I][q];
-#pragma unroll
-      for (int I = 0; I < 2; ++I)
-#pragma unroll
-        for (int J = 0; J < 2; ++J) {
-          v4d y = {0.0, 0.0, 0.0, 0.0}, yt = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-          for (int K = 0; K < 2; ++K)
+          for (int J = 0; J < NB; ++J)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-              y = __builtin_amdgcn_mfma_f64_16x16x4f64(miA[I][K][q], dmB[K][J][q], y, 0, 0, 0);
-              yt = __builtin_amdgcn_mfma_f64_16x16x4f64(dmB[K][I][q], miA[J][K][q], yt, 0, 0, 0);
+              const int rr = K * 16 + kq + 4 * q, cc = J * 16 + ri;
+              d[c][K][J][q] = (k < nd && rr < n && cc < n) ? Tg[rr * n + cc] : TA(0);
             }
-          qacc += y[0] * yt[0] + y[1] * yt[1] + y[2] * yt[2] + y[3] * yt[3];
-        }
-      gp = wave_sum_d(gp);
-      if (lane == 0 && valid_mat) g[bd * nd + k] = gp;
+      }
+    };
+    load(cur, 0);
+    for (int k0 = 0; k0 < nd; k0 += G) {
+      load(nxt, k0 + G);
+      // one channel at a time: only the loads of the next group overlap the products (register budget)
+      __builtin_amdgcn_sched_barrier(0);
+      double gp[G];
+#pragma unroll
+      for (int c = 0; c < G; ++c) {
+        double dmB[NB][NB][4];
+#pragma unroll
+        for (int K = 0; K < NB; ++K)
+#pragma unroll
+          for (int J = 0; J < NB; ++J)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) dmB[K][J][q] = (double)cur[c][K][J][q];
+        double t = 0.0;
+#pragma unroll
+        for (int I = 0; I < NB; ++I)
+#pragma unroll
+          for (int K = 0; K < NB; ++K)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) t += miA[I][K][q] * dmB[K][I][q];
+        gp[c] = t;
+#pragma unroll
+        for (int I = 0; I < NB; ++I)
+#pragma unroll
+          for (int J = 0; J < NB; ++J) {
+            v4d y = {0.0, 0.0, 0.0, 0.0}, yt = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int K = 0; K < NB; ++K)
+#pragma unroll
+              for (int q = 0; q < 4; ++q) {
+                y = __builtin_amdgcn_mfma_f64_16x16x4f64(miA[I][K][q], dmB[K][J][q], y, 0, 0, 0);
+                yt = __builtin_amdgcn_mfma_f64_16x16x4f64(dmB[K][I][q], miA[J][K][q], yt, 0, 0, 0);
+              }
+            qacc += y[0] * yt[0] + y[1] * yt[1] + y[2] * yt[2] + y[3] * yt[3];
+          }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      const double s = wave_sum_group<G>(gp, lane);
+      const int kc = k0 + lane / (64 / G);
+      if (lane % (64 / G) == 0 && kc < nd && valid_mat) g[bd * nd + kc] = s;
+#pragma unroll
+      for (int c = 0; c < G; ++c)
+#pragma unroll
+        for (int K = 0; K < NB; ++K)
+#pragma unroll
+          for (int J = 0; J < NB; ++J)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) cur[c][K][J][q] = nxt[c][K][J][q];
     }
     qacc = wave_sum_d(qacc);
   }
