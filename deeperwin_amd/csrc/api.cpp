@@ -350,7 +350,7 @@ size_t ws_bytes(const dw_model* m, long B, int C, size_t esz) {
   tot += (size_t)B * Cp * kmmax * esz + (size_t)B * Cp * m->W1 * esz * (C == 1 ? 4 : 1);
   if (esz == 8 && C > 1) tot += (size_t)rows * m->W1 * esz;  // unfused pre-activations (FP64 form)
   tot += (size_t)rows * m->W1 * esz + (size_t)rows * m->NO * esz;
-  tot += (size_t)B * m->D * C * m->n * m->n * esz;
+  if (!det_reads_orbitals(m->n, C)) tot += (size_t)B * m->D * C * m->n * m->n * esz;  // envelope x orbitals
   tot += (size_t)B * m->D * (8 * 2 + 4);
   if (C > 1) tot += (size_t)B * m->D * (3 * m->n + 1) * 8;
   return tot;
@@ -380,7 +380,7 @@ dw_status ensure_ws(dw_model* m, Workspace& w, long B, int C, size_t esz) {
     if (esz == 8 && C > 1) carve(w.Z, (size_t)rows * m->W1 * esz);
     carve(w.H, (size_t)rows * m->W1 * esz);
     carve(w.O, (size_t)rows * m->NO * esz);
-    carve(w.A, (size_t)B * m->D * C * m->n * m->n * esz);
+    if (!det_reads_orbitals(m->n, C)) carve(w.A, (size_t)B * m->D * C * m->n * m->n * esz);
     if (!fits)
       return fail(DW_ERR_INVALID, "the workspace given to dw_model_set_workspace is smaller than dw_workspace_size");
   }
@@ -392,7 +392,7 @@ dw_status ensure_ws(dw_model* m, Workspace& w, long B, int C, size_t esz) {
   if (esz == 8 && C > 1) HIP_TRY(w.Z.ensure((size_t)rows * m->W1 * esz));
   HIP_TRY(w.H.ensure((size_t)rows * m->W1 * esz));
   HIP_TRY(w.O.ensure((size_t)rows * m->NO * esz));
-  HIP_TRY(w.A.ensure((size_t)B * m->D * C * m->n * m->n * esz));
+  if (!det_reads_orbitals(m->n, C)) HIP_TRY(w.A.ensure((size_t)B * m->D * C * m->n * m->n * esz));
   HIP_TRY(m->ldd.ensure((size_t)B * m->D * 8));
   HIP_TRY(m->sdd.ensure((size_t)B * m->D * 8));
   HIP_TRY(m->dcond.ensure((size_t)B * m->D * 4));
@@ -531,11 +531,13 @@ dw_status run_pass(dw_model* m, const float* r, long B, const float* R, const fl
          m->W1, e0, s);
   gemm_t(m, ws.H.as<T>() + up_rows * m->W1, m->W1, W.Wo[1].as<T>(), W.Wo_hi[1], W.Wo_lo[1], m->W1,
          ws.O.as<T>() + up_rows * m->NO, m->NO, dn_rows, m->NO, m->W1, e0, s);
-  launch_envelope<T>(ws.O.as<T>(), m->NO, r, R, W.kappa.as<T>(), W.wenv.as<T>(), ws.A.as<T>(), sh, s);
+  const OrbSrc<T> src{ws.O.as<T>(), m->NO, r, R, W.kappa.as<T>(), W.wenv.as<T>()};
+  if (!det_reads_orbitals(m->n, C))
+    launch_envelope<T>(ws.O.as<T>(), m->NO, r, R, W.kappa.as<T>(), W.wenv.as<T>(), ws.A.as<T>(), sh, s);
   {
     Timed t(m, C > 1 ? DW_TIMER_DET_LAP : -1, s);
-    launch_det<T>(ws.A.as<T>(), m->ldd.as<double>(), m->sdd.as<double>(), m->gd.as<double>(), m->lapd.as<double>(),
-                  m->dcond.as<float>(), sh, s);
+    launch_det<T>(ws.A.as<T>(), src, m->ldd.as<double>(), m->sdd.as<double>(), m->gd.as<double>(),
+                  m->lapd.as<double>(), m->dcond.as<float>(), sh, s);
   }
   launch_combine(m->ldd.as<double>(), m->sdd.as<double>(), m->gd.as<double>(), m->lapd.as<double>(),
                  m->dcond.as<float>(), cond_thr, r, R, Z, out_lp, out_phase, out_eloc, out_flags,

@@ -68,9 +68,23 @@ void launch_tanh_rule(const T* Z, T* out, int ldo, int permute, const Shape& sh,
 template <typename T>
 void launch_envelope(const T* O, int ldo, const float* r, const float* R, const T* kappa, const T* wenv, T* A,
                      const Shape& sh, hipStream_t s);
+// Orbitals and envelope parameters as the determinant kernels read them when they apply the envelope
+// themselves (det_reads_orbitals): O holds spin-major rows (Cp per electron, ldo columns).
+template <typename T>
+struct OrbSrc {
+  const T* O;
+  int ldo;
+  const float* r;
+  const float* R;
+  const T* kappa;
+  const T* wenv;
+};
+// True when the determinant kernels for n electrons and C channels apply the envelope while reading O
+// (no envelope launch, no stored product); otherwise launch_envelope fills A first.
+inline bool det_reads_orbitals(int n, int C) { return C == 1 ? n <= 32 : n <= 16; }
 template <typename TA>
-void launch_det(const TA* A, double* ld, double* sd, double* g, double* lapd, float* dcond, const Shape& sh,
-                hipStream_t s);
+void launch_det(const TA* A, const OrbSrc<TA>& src, double* ld, double* sd, double* g, double* lapd, float* dcond,
+                const Shape& sh, hipStream_t s);
 void launch_combine(const double* ld, const double* sd, const double* g, const double* lapd, const float* dcond,
                     float cond_thr, const float* r, const float* R, const float* Z, float* out_lp, float* out_phase,
                     float* out_eloc, uint32_t* out_flags, float* out_kappa, const Shape& sh, hipStream_t s);

@@ -660,6 +660,44 @@ template void launch_tanh_rule<double>(const double*, double*, int, int, const S
 //   Lap A = Lap(env) bf + 2 grad_i(env) . grad_i(bf) + env Lap(bf),   Lap e^{-k rho} = (k^2 - 2k/rho) e^{-k rho}.
 // O rows are spin-major (up block then down block, Cp rows per electron); A is [B, D, C, n, n].
 // ------------------------------------------------------------------------------------------------
+// Spin-major row of electron i's first channel in O (up block, then down block, Cp rows per electron).
+__device__ __forceinline__ long orb_row(const Shape& sh, long b, int i) {
+  const int dn = sh.n - sh.u;
+  return (i < sh.u) ? (b * sh.u + i) * sh.Cp : (long)sh.B * sh.u * sh.Cp + (b * dn + (i - sh.u)) * sh.Cp;
+}
+
+// Envelope of electron i of walker b at orbital column col; with GRAD also its gradient in r_i and its
+// Laplacian.  Shared by the envelope kernel and the determinant kernels that apply it while reading O.
+template <typename T, bool GRAD>
+__device__ __forceinline__ void envelope_at(const T* __restrict__ kappa, const T* __restrict__ wenv,
+                                            const float* __restrict__ r, const float* __restrict__ R, const Shape& sh,
+                                            long b, int i, int col, T& env, T& g0, T& g1, T& g2, T& lenv) {
+  const int NO = sh.D * sh.n, s = (i < sh.u) ? 0 : 1;
+  const float* ri = r + (b * sh.n + i) * 3;
+  const T* kap = kappa + (long)s * sh.I * NO;
+  const T* we = wenv + (long)s * sh.I * NO;
+  env = T(0);
+  g0 = T(0);
+  g1 = T(0);
+  g2 = T(0);
+  lenv = T(0);
+#pragma unroll 1
+  for (int I = 0; I < sh.I; ++I) {
+    const T dx = (T)ri[0] - (T)R[I * 3 + 0], dy = (T)ri[1] - (T)R[I * 3 + 1], dz = (T)ri[2] - (T)R[I * 3 + 2];
+    const T rho = dsqrt(dx * dx + dy * dy + dz * dz);
+    const T k = kap[I * NO + col];
+    const T e = we[I * NO + col] * dexp(-k * rho);
+    env += e;
+    if constexpr (GRAD) {
+      const T coef = -k * e / rho;
+      g0 += coef * dx;
+      g1 += coef * dy;
+      g2 += coef * dz;
+      lenv += e * (k * k - T(2) * k / rho);
+    }
+  }
+}
+
 template <typename T>
 __global__ void k_envelope(const T* __restrict__ O, int ldo, const float* __restrict__ r, const float* __restrict__ R,
                            const T* __restrict__ kappa, const T* __restrict__ wenv, T* __restrict__ A, Shape sh) {
@@ -675,29 +713,12 @@ __global__ void k_envelope(const T* __restrict__ O, int ldo, const float* __rest
   const long b = div_idx(bd_, sh.D);
   const int d_ = (int)(bd_ - b * sh.D);
   const int col = d_ * n + kk_;
-  const long bi = b * n + i;
-  const int s = (i < sh.u) ? 0 : 1;
-  const int dn = n - sh.u;
-  const int Cp = sh.Cp;
-  const long orow = (s == 0) ? (b * sh.u + i) * Cp : (long)sh.B * sh.u * Cp + (b * dn + (i - sh.u)) * Cp;
-  const float* ri = r + bi * 3;
-  const T* kap = kappa + (long)s * sh.I * NO;
-  const T* we = wenv + (long)s * sh.I * NO;
-  T env = T(0), g0 = T(0), g1 = T(0), g2 = T(0), lenv = T(0);
-  for (int I = 0; I < sh.I; ++I) {
-    const T dx = (T)ri[0] - (T)R[I * 3 + 0], dy = (T)ri[1] - (T)R[I * 3 + 1], dz = (T)ri[2] - (T)R[I * 3 + 2];
-    const T rho = dsqrt(dx * dx + dy * dy + dz * dz);
-    const T k = kap[I * NO + col];
-    const T e = we[I * NO + col] * dexp(-k * rho);
-    env += e;
-    if (C > 1) {
-      const T coef = -k * e / rho;
-      g0 += coef * dx;
-      g1 += coef * dy;
-      g2 += coef * dz;
-      lenv += e * (k * k - T(2) * k / rho);
-    }
-  }
+  const long orow = orb_row(sh, b, i);
+  T env, g0, g1, g2, lenv;
+  if (C > 1)
+    envelope_at<T, true>(kappa, wenv, r, R, sh, b, i, col, env, g0, g1, g2, lenv);
+  else
+    envelope_at<T, false>(kappa, wenv, r, R, sh, b, i, col, env, g0, g1, g2, lenv);
   const int d = col / n, k = col % n;
   const long nn = (long)n * n;
   T* Ab = A + ((b * sh.D + d) * C) * nn + (long)i * n + k;
@@ -717,10 +738,10 @@ __global__ void k_envelope(const T* __restrict__ O, int ldo, const float* __rest
       for (int q = 0; q < 8; ++q) Ac[(long)(ch + q) * nn] = v[q];
     }
     for (; ch < nt; ++ch) Ac[(long)ch * nn] = env * Oc[(long)ch * ldo];
-    // electron i's own coordinates also carry the envelope gradient
-    Ac[(long)(3 * i + 0) * nn] = env * Oc[(long)(3 * i + 0) * ldo] + g0 * bf0;
-    Ac[(long)(3 * i + 1) * nn] = env * Oc[(long)(3 * i + 1) * ldo] + g1 * bf0;
-    Ac[(long)(3 * i + 2) * nn] = env * Oc[(long)(3 * i + 2) * ldo] + g2 * bf0;
+    // electron i's own coordinates also carry the envelope gradient (one rounding: fma with g_c bf0)
+    Ac[(long)(3 * i + 0) * nn] = fma(env, Oc[(long)(3 * i + 0) * ldo], g0 * bf0);
+    Ac[(long)(3 * i + 1) * nn] = fma(env, Oc[(long)(3 * i + 1) * ldo], g1 * bf0);
+    Ac[(long)(3 * i + 2) * nn] = fma(env, Oc[(long)(3 * i + 2) * ldo], g2 * bf0);
     const T t0 = O[(orow + 1 + 3 * i) * ldo + col];
     const T t1 = O[(orow + 2 + 3 * i) * ldo + col];
     const T t2 = O[(orow + 3 + 3 * i) * ldo + col];
@@ -848,19 +869,35 @@ __device__ __forceinline__ void finish_slogdet(double mydiag, int swaps, bool si
 // column j), 4 wavefronts per workgroup, no LDS.  Pivot search in the pivot column's lane, the pivot
 // index, pivot and multipliers are broadcast inside the group with shuffles.
 template <int NMAX, typename TA>
-__global__ __launch_bounds__(256) void k_det_value(const TA* __restrict__ A, double* __restrict__ ld,
-                                                   double* __restrict__ sd, float* __restrict__ dcond, long nmat,
-                                                   int n, long stride) {
+__global__ __launch_bounds__(256) void k_det_value(OrbSrc<TA> src, double* __restrict__ ld, double* __restrict__ sd,
+                                                   float* __restrict__ dcond, long nmat, int n, Shape sh) {
   constexpr int G = 64 / NMAX;
   const int lane = threadIdx.x & 63;
   const int g = lane / NMAX, j = lane % NMAX, base = g * NMAX;
   const long bd = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * G + g;
   const bool valid = bd < nmat;
-  const TA* Ab = A + (valid ? bd : nmat - 1) * stride;  // out-of-range groups factor a copy, never stored
+  const long bdl = valid ? bd : nmat - 1;  // out-of-range groups factor a copy, never stored
+  const long b = bdl / sh.D;
+  const int oc = (int)(bdl - b * sh.D) * n + j;  // orbital column of matrix column j
   const bool act = j < n;
+  // matrix element (i, j) = env(r_i, column) * O[electron i, column]: the envelope is applied as O is read.
+  // Each lane forms its column in a rolled loop into wave-private LDS (lane-contiguous, conflict-free) and
+  // reads it back into registers: the element phase stays out of the unrolled factorisation.
+  __shared__ TA sA[4][NMAX][64];
+  TA (*mine)[64] = sA[threadIdx.x >> 6];
+#pragma unroll 1
+  for (int i = 0; i < n; ++i) {
+    TA v = TA(0);
+    if (act) {
+      TA env, g0, g1, g2, lv;
+      envelope_at<TA, false>(src.kappa, src.wenv, src.r, src.R, sh, b, i, oc, env, g0, g1, g2, lv);
+      v = env * src.O[orb_row(sh, b, i) * src.ldo + oc];
+    }
+    mine[i][lane] = v;
+  }
   double col[NMAX];
 #pragma unroll
-  for (int i = 0; i < NMAX; ++i) col[i] = (act && i < n) ? (double)Ab[i * n + j] : 0.0;
+  for (int i = 0; i < NMAX; ++i) col[i] = (act && i < n) ? (double)mine[i][lane] : 0.0;
   int swaps = 0;
   bool sing = false;
   double mydiag = 1.0;
@@ -929,29 +966,67 @@ __global__ __launch_bounds__(256) void k_det_value(const TA* __restrict__ A, dou
 
 // Laplacian mode (n <= 32): WPB matrices per workgroup, one per wavefront; LU and inverse with a lane per
 // column, M^-1 through LDS into MFMA operand registers, then the channel passes on FP64 MFMA.
-template <int NMAX, int WPB, typename TA>
-__global__ __launch_bounds__(64 * WPB) void k_det_lap(const TA* __restrict__ A, double* __restrict__ ld,
-                                                      double* __restrict__ sd, double* __restrict__ g,
-                                                      double* __restrict__ lapd, float* __restrict__ dcond, long nmat,
-                                                      int n, int C) {
+// ENV: the matrices are read from the orbitals O and the envelope is applied on the way (no stored A);
+// while the value channel is read, each wavefront tabulates in LDS the envelope, the own-electron
+// gradient terms g_c bf0 (channel 3i + c of row i gains them) and the Laplacian channel, [row][column].
+template <int NMAX, int WPB, typename TA, bool ENV>
+__global__ __launch_bounds__(64 * WPB) void k_det_lap(const TA* __restrict__ A, OrbSrc<TA> src, Shape sh,
+                                                      double* __restrict__ ld, double* __restrict__ sd,
+                                                      double* __restrict__ g, double* __restrict__ lapd,
+                                                      float* __restrict__ dcond, long nmat, int n, int C) {
   extern __shared__ double dsm[];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   long bd = (long)blockIdx.x * WPB + wv;
   const bool valid_mat = bd < nmat;
   if (!valid_mat) bd = nmat - 1;  // keep the wave in the barriers; results are not stored
   static_assert(NMAX == 16 || NMAX == 32, "channel passes are written for 16 x 16 MFMA blocks");
-  double* sMi = dsm + (size_t)wv * (NMAX * NMAX);
+  constexpr int NN = NMAX * NMAX;
+  double* sMi = dsm + (size_t)wv * NN;
+  TA* sE = (TA*)(dsm + (size_t)WPB * NN) + (size_t)wv * 5 * NN;  // ENV tables: [row][column]
+  TA* sG = sE + NN;                                               // [c][row][column]
+  TA* sL = sE + 4 * NN;
   const long nn = (long)n * n;
-  const TA* Ab = A + bd * C * nn;
+  const TA* Ab = ENV ? nullptr : A + bd * C * nn;
   const bool act = lane < n;
   const int nd = 3 * n;
+  const long b = bd / sh.D;
+  const int dcol = (int)(bd - b * sh.D) * n;  // first orbital column of this determinant
 
   double col[NMAX];
   int perm[NMAX];
   double csum = 0.0;
 #pragma unroll
   for (int i = 0; i < NMAX; ++i) {
-    col[i] = (act && i < n) ? (double)Ab[i * n + lane] : 0.0;
+    if constexpr (ENV) {
+      double v = 0.0;
+      TA e0 = TA(0), gb0 = TA(0), gb1 = TA(0), gb2 = TA(0), lp = TA(0);
+      if (act && i < n) {
+        const int oc = dcol + lane;
+        TA env, g0, g1, g2, lv;
+        envelope_at<TA, true>(src.kappa, src.wenv, src.r, src.R, sh, b, i, oc, env, g0, g1, g2, lv);
+        const TA* Oi = src.O + orb_row(sh, b, i) * src.ldo + oc;
+        const TA bf0 = Oi[0];
+        const TA t0 = Oi[(long)(1 + 3 * i) * src.ldo];
+        const TA t1 = Oi[(long)(2 + 3 * i) * src.ldo];
+        const TA t2 = Oi[(long)(3 + 3 * i) * src.ldo];
+        v = (double)(env * bf0);
+        e0 = env;
+        gb0 = g0 * bf0;
+        gb1 = g1 * bf0;
+        gb2 = g2 * bf0;
+        lp = lv * bf0 + env * Oi[(long)(C - 1) * src.ldo] + TA(2) * (g0 * t0 + g1 * t1 + g2 * t2);
+      }
+      if (lane < NMAX) {  // zero outside n x n
+        sE[i * NMAX + lane] = e0;
+        sG[(0 * NMAX + i) * NMAX + lane] = gb0;
+        sG[(1 * NMAX + i) * NMAX + lane] = gb1;
+        sG[(2 * NMAX + i) * NMAX + lane] = gb2;
+        sL[i * NMAX + lane] = lp;
+      }
+      col[i] = v;
+    } else {
+      col[i] = (act && i < n) ? (double)Ab[i * n + lane] : 0.0;
+    }
     csum += fabs(col[i]);
   }
   const double normA = wave_max_d(csum);
@@ -1013,12 +1088,25 @@ __global__ __launch_bounds__(64 * WPB) void k_det_lap(const TA* __restrict__ A, 
       for (int K = 0; K < NB; ++K)
 #pragma unroll
         for (int q = 0; q < 4; ++q) miA[I][K][q] = sMi[(I * 16 + ri) * NMAX + K * 16 + kq + 4 * q];
+    // ENV: envelope and the O row base of each element this lane holds
+    TA envq[NB][NB][4];
+    const TA* ob[NB][NB][4];
+#pragma unroll
+    for (int K = 0; K < NB; ++K)
+#pragma unroll
+      for (int J = 0; J < NB; ++J)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int rr = K * 16 + kq + 4 * q, cc = J * 16 + ri;
+          envq[K][J][q] = ENV ? sE[rr * NMAX + cc] : TA(1);
+          ob[K][J][q] = (ENV && rr < n && cc < n) ? src.O + orb_row(sh, b, rr) * src.ldo + dcol + cc : src.O;
+        }
     TA cur[G][NB][NB][4], nxt[G][NB][NB][4];
     auto load = [&](TA (&d)[G][NB][NB][4], int k0) {
 #pragma unroll
       for (int c = 0; c < G; ++c) {
         const int k = k0 + c;
-        const TA* Tg = Ab + (long)(1 + (k < nd ? k : 0)) * nn;
+        const TA* Tg = ENV ? nullptr : Ab + (long)(1 + (k < nd ? k : 0)) * nn;
 #pragma unroll
         for (int K = 0; K < NB; ++K)
 #pragma unroll
@@ -1026,7 +1114,11 @@ __global__ __launch_bounds__(64 * WPB) void k_det_lap(const TA* __restrict__ A, 
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
               const int rr = K * 16 + kq + 4 * q, cc = J * 16 + ri;
-              d[c][K][J][q] = (k < nd && rr < n && cc < n) ? Tg[rr * n + cc] : TA(0);
+              const bool in = k < nd && rr < n && cc < n;
+              if constexpr (ENV)
+                d[c][K][J][q] = in ? ob[K][J][q][(long)(1 + k) * src.ldo] : TA(0);
+              else
+                d[c][K][J][q] = in ? Tg[rr * n + cc] : TA(0);
             }
       }
     };
@@ -1039,12 +1131,21 @@ __global__ __launch_bounds__(64 * WPB) void k_det_lap(const TA* __restrict__ A, 
 #pragma unroll
       for (int c = 0; c < G; ++c) {
         double dmB[NB][NB][4];
+        const int kr = (k0 + c) / 3, kc3 = (k0 + c) - 3 * kr;  // the channel's electron and component
 #pragma unroll
         for (int K = 0; K < NB; ++K)
 #pragma unroll
           for (int J = 0; J < NB; ++J)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) dmB[K][J][q] = (double)cur[c][K][J][q];
+            for (int q = 0; q < 4; ++q) {
+              if constexpr (ENV) {
+                const int rr = K * 16 + kq + 4 * q, cc = J * 16 + ri;
+                const TA own = (rr == kr) ? sG[(kc3 * NMAX + rr) * NMAX + cc] : TA(0);
+                dmB[K][J][q] = (double)fma(envq[K][J][q], cur[c][K][J][q], own);
+              } else {
+                dmB[K][J][q] = (double)cur[c][K][J][q];
+              }
+            }
         double t = 0.0;
 #pragma unroll
         for (int I = 0; I < NB; ++I)
@@ -1084,12 +1185,12 @@ __global__ __launch_bounds__(64 * WPB) void k_det_lap(const TA* __restrict__ A, 
     qacc = wave_sum_d(qacc);
   }
   // tr(M^-1 Lap M): lane c takes column c of Lap M and row c of M^-1
-  const TA* Tl = Ab + (long)(C - 1) * nn;
+  const TA* Tl = ENV ? nullptr : Ab + (long)(C - 1) * nn;
   double tp = 0.0;
   if (act) {
 #pragma unroll
     for (int e = 0; e < NMAX; ++e)
-      if (e < n) tp += sMi[lane * NMAX + e] * (double)Tl[e * n + lane];
+      if (e < n) tp += sMi[lane * NMAX + e] * (double)(ENV ? sL[e * NMAX + lane] : Tl[e * n + lane]);
   }
   tp = wave_sum_d(tp);
   if (lane == 0 && valid_mat) {
@@ -1395,8 +1496,8 @@ __global__ __launch_bounds__(64) void k_det_value_lds(const TA* __restrict__ A, 
 }
 
 template <typename TA>
-void launch_det(const TA* A, double* ld, double* sd, double* g, double* lapd, float* dcond, const Shape& sh,
-                hipStream_t s) {
+void launch_det(const TA* A, const OrbSrc<TA>& src, double* ld, double* sd, double* g, double* lapd, float* dcond,
+                const Shape& sh, hipStream_t s) {
   const int n = sh.n;
   const long nmat = (long)sh.B * sh.D;
   const long stride = (long)sh.C * n * n;
@@ -1414,32 +1515,35 @@ void launch_det(const TA* A, double* ld, double* sd, double* g, double* lapd, fl
     return;
   }
   if (sh.C == 1) {
-    // 4 wavefronts per workgroup, 64 / NMAX matrices per wavefront
+    // 4 wavefronts per workgroup, 64 / NMAX matrices per wavefront; the envelope is applied as O is read
     if (n <= 8)
-      hipLaunchKernelGGL((k_det_value<8, TA>), dim3((unsigned)((nmat + 31) / 32)), dim3(256), 0, s, A, ld, sd, dcond,
-                         nmat, n, stride);
+      hipLaunchKernelGGL((k_det_value<8, TA>), dim3((unsigned)((nmat + 31) / 32)), dim3(256), 0, s, src, ld, sd,
+                         dcond, nmat, n, sh);
     else if (n <= 16)
-      hipLaunchKernelGGL((k_det_value<16, TA>), dim3((unsigned)((nmat + 15) / 16)), dim3(256), 0, s, A, ld, sd, dcond,
-                         nmat, n, stride);
+      hipLaunchKernelGGL((k_det_value<16, TA>), dim3((unsigned)((nmat + 15) / 16)), dim3(256), 0, s, src, ld, sd,
+                         dcond, nmat, n, sh);
     else
-      hipLaunchKernelGGL((k_det_value<32, TA>), dim3((unsigned)((nmat + 7) / 8)), dim3(256), 0, s, A, ld, sd, dcond,
-                         nmat, n, stride);
+      hipLaunchKernelGGL((k_det_value<32, TA>), dim3((unsigned)((nmat + 7) / 8)), dim3(256), 0, s, src, ld, sd,
+                         dcond, nmat, n, sh);
     return;
   }
   if (n <= 16) {
+    // M^-1 (FP64) plus the envelope tables (5 [16][16] arrays of TA) per wavefront
     constexpr int NM = 16, W = 4;
-    const size_t shm = (size_t)W * (NM * NM) * sizeof(double);
-    hipLaunchKernelGGL((k_det_lap<NM, W, TA>), dim3((unsigned)((nmat + W - 1) / W)), dim3(64 * W), shm, s, A, ld, sd,
-                       g, lapd, dcond, nmat, n, sh.C);
+    const size_t shm = (size_t)W * (NM * NM) * (sizeof(double) + 5 * sizeof(TA));
+    hipLaunchKernelGGL((k_det_lap<NM, W, TA, true>), dim3((unsigned)((nmat + W - 1) / W)), dim3(64 * W), shm, s, A,
+                       src, sh, ld, sd, g, lapd, dcond, nmat, n, sh.C);
   } else {
     constexpr int NM = 32, W = 2;
     const size_t shm = (size_t)W * (NM * NM) * sizeof(double);
-    hipLaunchKernelGGL((k_det_lap<NM, W, TA>), dim3((unsigned)((nmat + W - 1) / W)), dim3(64 * W), shm, s, A, ld, sd,
-                       g, lapd, dcond, nmat, n, sh.C);
+    hipLaunchKernelGGL((k_det_lap<NM, W, TA, false>), dim3((unsigned)((nmat + W - 1) / W)), dim3(64 * W), shm, s, A,
+                       src, sh, ld, sd, g, lapd, dcond, nmat, n, sh.C);
   }
 }
-template void launch_det<float>(const float*, double*, double*, double*, double*, float*, const Shape&, hipStream_t);
-template void launch_det<double>(const double*, double*, double*, double*, double*, float*, const Shape&, hipStream_t);
+template void launch_det<float>(const float*, const OrbSrc<float>&, double*, double*, double*, double*, float*,
+                                const Shape&, hipStream_t);
+template void launch_det<double>(const double*, const OrbSrc<double>&, double*, double*, double*, double*, float*,
+                                 const Shape&, hipStream_t);
 
 // ------------------------------------------------------------------------------------------------
 // Per-walker combination (FP64).  Sum of determinants with max shift and LOG_EPSILON
