@@ -44,12 +44,13 @@ struct Shape {
 };
 
 // Fused 1-el GEMM + tanh forward-Laplacian rule of the Laplacian pass (rows: Cp per electron).
-// E_override > 0 sets the electrons per workgroup (E * Cp / 16 <= 12); 0 picks E = max(1, 96 / Cp).
+// E_override > 0 sets the electrons per workgroup (E * Cp / 16 <= 12); 0 picks lap_electrons_per_tile:
+// FP32 min(4, 192 / Cp), BF16 forms 96 / Cp (at least 1).
 // cols = 256 asks for 256-column tiles (FP32 form, E * Cp <= 96); otherwise 128.
 void gemm_lap(GemmPrec prec, const float* X, int ldx, const float* Bt, const uint16_t* Bhi, const uint16_t* Blo, int K,
               const float* V, int ldv, float* out, int ldo, int permute, const Shape& sh, int E_override, int cols,
               hipStream_t s);
-int lap_electrons_per_tile(int Cp);
+int lap_electrons_per_tile(int Cp, GemmPrec prec);
 
 // Value-pass 1-el GEMM on the Laplacian GEMM's 16x16 MFMA main loop (FP32): out = tanh(X W + V[walker]),
 // V summed over vsplit parts vstride apart; permute sends rows to spin-major order.

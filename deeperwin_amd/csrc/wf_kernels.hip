@@ -583,12 +583,24 @@ __global__ void k_spin_mean(const T* __restrict__ X, int Kx, int Kh, T* __restri
     const int i0 = s ? sh.u : 0, i1 = s ? sh.n : sh.u;
     const T* xp = X + ((b * sh.n + i0) * sh.Cp + ch) * (long)Kx + fi;
     const long stride = (long)sh.Cp * Kx;
+    if constexpr (sizeof(T) == 4) {
+      // one 16-byte load per electron (rows and fi are multiples of 4 floats)
 #pragma unroll 4
-    for (int i = i0; i < i1; ++i, xp += stride) {
-      v0 += xp[0];
-      v1 += xp[1];
-      v2 += xp[2];
-      v3 += xp[3];
+      for (int i = i0; i < i1; ++i, xp += stride) {
+        const float4 x = *(const float4*)xp;
+        v0 += x.x;
+        v1 += x.y;
+        v2 += x.z;
+        v3 += x.w;
+      }
+    } else {
+#pragma unroll 4
+      for (int i = i0; i < i1; ++i, xp += stride) {
+        v0 += xp[0];
+        v1 += xp[1];
+        v2 += xp[2];
+        v3 += xp[3];
+      }
     }
     const T inv = T(1) / (T)(i1 - i0);
     v0 *= inv;
