@@ -53,8 +53,8 @@ def _tol_e(ref, kappa):
     return (1.0 + np.abs(ref)) * (1e-4 + 4 * U32 * kappa)
 
 
-@pytest.mark.parametrize("name,B", [("lih", 64), ("li", 32), ("n2", 32), ("odd17", 8), ("c4h2", 8), ("odd33", 4),
-                                    ("big", 8), ("big60", 4)])
+@pytest.mark.parametrize("name,B", [("lih", 64), ("li", 32), ("n2", 32), ("mol16", 8), ("odd17", 8), ("c4h2", 8),
+                                    ("mol32", 4), ("odd33", 4), ("big", 8), ("big60", 4)])
 def test_logpsi_matches_oracle(cuda, name, B):
     shape, params, R, Z, mapping = system(name)
     r = walkers(R, mapping, B, seed=1)
@@ -74,8 +74,8 @@ def test_logpsi_matches_oracle(cuda, name, B):
     np.testing.assert_array_equal(ph.cpu().numpy(), ph_ref.numpy().astype(np.float32))
 
 
-@pytest.mark.parametrize("name,B", [("lih", 32), ("li", 16), ("n2", 8), ("odd17", 4), ("c4h2", 4), ("odd33", 2),
-                                    ("big", 2), ("big60", 2)])
+@pytest.mark.parametrize("name,B", [("lih", 32), ("li", 16), ("n2", 8), ("mol16", 4), ("odd17", 4), ("c4h2", 4),
+                                    ("mol32", 2), ("odd33", 2), ("big", 2), ("big60", 2)])
 def test_local_energy_matches_oracle(cuda, name, B):
     shape, params, R, Z, mapping = system(name)
     r = walkers(R, mapping, B, seed=2)
@@ -133,7 +133,7 @@ def test_mcmc_replay_matches_oracle(cuda):
     assert n == S and abs(s - 0.02) < 1e-7
 
 
-@pytest.mark.parametrize("name,B", [("lih", 16), ("n2", 4), ("big", 2), ("big60", 2)])
+@pytest.mark.parametrize("name,B", [("lih", 16), ("n2", 4), ("mol16", 2), ("big", 2), ("big60", 2)])
 def test_fp64_form_matches_oracle(cuda, name, B):
     """The whole pass in FP64 (the reference's float64 mode): only the float32 outputs round."""
     shape, params, R, Z, mapping = system(name)

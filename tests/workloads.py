@@ -30,7 +30,9 @@ def system(name: str, n_det: int = None, seed: int = 0):
     elif name == "li":
         R, Z = np.zeros((1, 3)), np.array([3])
         D = n_det or 3
-    elif name in ("odd17", "odd33"):
+    elif name in ("odd17", "odd33", "mol16", "mol32"):
+        # mol16 / mol32 fill the 16- and 32-wide determinant tiles exactly (the limits of the kernels that
+        # apply the envelope while reading the orbitals in the Laplacian and value passes)
         from deeperwin_amd.physics import random_molecule
 
         k = int(name[3:])
