@@ -31,41 +31,44 @@ __device__ __forceinline__ double dexp(double x) { return exp(x); }
 template <typename T>
 __global__ void k_features0(const float* __restrict__ r, const float* __restrict__ R, T* __restrict__ X0, int Kx0,
                             Shape sh) {
-  const long total = (long)sh.B * sh.n * sh.Cp * Kx0;
+  // one thread per (electron row bi, column): the distance to the column's ion is formed once and the
+  // column is written for every channel row (consecutive threads write consecutive columns of a row)
+  const long total = (long)sh.B * sh.n * Kx0;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
-  const long row = div_idx(t, Kx0);
-  const int col = (int)(t - row * Kx0);
-  const long bi = div_idx(row, sh.Cp);
-  const int ch = (int)(row - bi * sh.Cp);
+  const long bi = div_idx(t, Kx0);
+  const int col = (int)(t - bi * Kx0);
   const int i = (int)(bi - div_idx(bi, sh.n) * sh.n);
-  const int I4 = 4 * sh.I;
-  if (ch >= sh.C) {  // padding channel rows stay zero
-    X0[t] = T(0);
+  const int I4 = 4 * sh.I, C = sh.C, Cp = sh.Cp;
+  T* out = X0 + bi * Cp * (long)Kx0 + col;
+  if (col >= I4) {
+    // pair-mean columns are written by k_twoel for the real channels; padding columns are zero
+    for (int ch = (col < I4 + 8) ? C : 0; ch < Cp; ++ch) out[(long)ch * Kx0] = T(0);
     return;
   }
-  if (col >= I4 && col < I4 + 8) return;  // pair-mean columns: written by k_twoel
-  T v = T(0);
-  if (col < I4) {
-    const int ion = col >> 2, f = col & 3;
-    const float* ri = r + bi * 3;
-    const T d[3] = {(T)ri[0] - (T)R[ion * 3 + 0], (T)ri[1] - (T)R[ion * 3 + 1], (T)ri[2] - (T)R[ion * 3 + 2]};
-    const T rho = dsqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    if (ch == 0) {
-      v = (f == 0) ? rho : d[f - 1];
-    } else if (ch == sh.C - 1) {
-      v = (f == 0) ? T(2) / rho : T(0);
-    } else {
-      const int k = ch - 1, e = k / 3, c = k % 3;
-      if (e == i) v = (f == 0) ? d[c] / rho : ((f - 1 == c) ? T(1) : T(0));
-    }
+  const int ion = col >> 2, f = col & 3;
+  const float* ri = r + bi * 3;
+  const T d[3] = {(T)ri[0] - (T)R[ion * 3 + 0], (T)ri[1] - (T)R[ion * 3 + 1], (T)ri[2] - (T)R[ion * 3 + 2]};
+  const T rho = dsqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  out[0] = (f == 0) ? rho : d[f - 1];
+  if (C == 1) {
+    for (int ch = 1; ch < Cp; ++ch) out[(long)ch * Kx0] = T(0);
+    return;
   }
-  X0[t] = v;
+  // tangent rows: only electron i's own three coordinates are non-zero
+  for (int ch = 1; ch < C - 1; ++ch) {
+    const int k = ch - 1, e = k / 3, c = k - 3 * e;
+    T v = T(0);
+    if (e == i) v = (f == 0) ? d[c] / rho : ((f - 1 == c) ? T(1) : T(0));
+    out[(long)ch * Kx0] = v;
+  }
+  out[(long)(C - 1) * Kx0] = (f == 0) ? T(2) / rho : T(0);  // Laplacian of |x| in 3D is 2/|x|
+  for (int ch = C; ch < Cp; ++ch) out[(long)ch * Kx0] = T(0);
 }
 
 template <typename T>
 void launch_features0(const float* r, const float* R, T* X0, int Kx0, const Shape& sh, hipStream_t s) {
-  const long total = (long)sh.B * sh.n * sh.Cp * Kx0;
+  const long total = (long)sh.B * sh.n * Kx0;
   hipLaunchKernelGGL(k_features0<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, r, R, X0, Kx0, sh);
 }
 template void launch_features0<float>(const float*, const float*, float*, int, const Shape&, hipStream_t);
