@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--requests", type=int, default=24, help="serving: requests in the stream")
     ap.add_argument("--rate", type=float, default=50.0, help="serving: offered load, requests/s")
     ap.add_argument("--fp64-threshold", type=float, default=1e6)
+    ap.add_argument("--eager", action="store_true",
+                    help="n2: launch the MCMC steps eagerly instead of replaying them as one HIP graph")
     ap.add_argument("--tune", action="store_true",
                     help="n2: pick launch shapes with the autotuner first (default: the built-in heuristics, so "
                          "every run times the same kernels)")
@@ -376,7 +378,7 @@ def main():
         sys.exit(2)
     import torch
 
-    from deeperwin_amd._native import TIMER_FORWARD, TIMER_GEMM_1EL_LAP, TIMER_LOCAL_ENERGY, NativeModel
+    from deeperwin_amd._native import TIMER_GEMM_1EL_LAP, TIMER_LOCAL_ENERGY, NativeModel
     from deeperwin_amd.mcmc import make_chain
     from deeperwin_amd.params import ModelShape, init_params
     from deeperwin_amd.physics import generate_el_ion_mapping, geometry_n2, init_walkers_gaussian
@@ -412,29 +414,56 @@ def main():
 
         tuning, _ = autotune_model(model, r, Rd, Zd, reps=3)
 
-    def step():
-        normals = torch.randn((S, B, n, 3), generator=gen, device=dev)
-        uniforms = torch.rand((S, B), generator=gen, device=dev)
+    # proposal normals and accept uniforms live in fixed buffers, refilled in place before every unit, so
+    # the S MCMC steps (one dw_mcmc_steps call, ~21 launches per step) can be replayed as one HIP graph
+    normals = torch.empty((S, B, n, 3), device=dev)
+    uniforms = torch.empty((S, B), device=dev)
+
+    def mcmc():
         model.mcmc_steps(r, Rd, Zd, lp, age, normals, uniforms, chain, 100, 100, 0.5, 1e-2, 1.0, out_acc=acc)
+
+    graph = None
+    mcmc_events = []
+
+    def step(record=False):
+        normals.normal_(generator=gen)
+        uniforms.uniform_(generator=gen)
+        if record:
+            a = torch.cuda.Event(enable_timing=True)
+            a.record()
+        if graph is not None:
+            graph.replay()
+        else:
+            mcmc()
+        if record:
+            b = torch.cuda.Event(enable_timing=True)
+            b.record()
+            mcmc_events.append((a, b))
         model.local_energy(r, Rd, Zd, e, None, flags)
 
-    for _ in range(args.warmup):
+    # graph mode needs one eager unit first (it sizes the workspace that the graph then reuses)
+    for _ in range(args.warmup if args.eager else max(args.warmup, 1)):
         step()
     torch.cuda.synchronize()
+    if not args.eager:
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            mcmc()
+        torch.cuda.synchronize()
     model.set_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step()
+        step(record=True)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     dt = (t1 - t0) / args.steps
     value = B / dt
 
     gemm_ms, gemm_n = model.timer(TIMER_GEMM_1EL_LAP)
-    fwd_ms, fwd_n = model.timer(TIMER_FORWARD)
     lap_ms, lap_n = model.timer(TIMER_LOCAL_ENERGY)
     model.set_timing(False)
+    mcmc_ms = float(np.mean([a.elapsed_time(b) for a, b in mcmc_events]))
     C = 3 * n + 2
     M = B * n * C
     K = shape.width1 + 2 * shape.width2
@@ -455,11 +484,12 @@ def main():
                                f"16 full dets, random init seed 1; {B} walkers; {S} MCMC steps (stepsize 0.02) + "
                                f"E_loc with the full Laplacian over {3 * n} directions per step",
                    "walkers": B, "mcmc_steps": S, "n_el": n, "n_det": D, "precision": args.precision,
-                   "parallelism": "single GPU (walker-parallel)", "launch_shapes": tuning or "heuristic"},
+                   "parallelism": "single GPU (walker-parallel)", "launch_shapes": tuning or "heuristic",
+                   "mcmc_launch": "eager" if args.eager else f"one HIP graph replay for the {S} MCMC steps"},
         "roofline": roofline,
-        "breakdown": {"forward_pass_ms": round(fwd_ms / max(fwd_n, 1), 4),
+        "breakdown": {"mcmc_step_ms": round(mcmc_ms / S, 4),
                       "local_energy_pass_ms": round(lap_ms / max(lap_n, 1), 4),
-                      "mcmc_walker_steps_per_s": round(B * S / ((S + 1) * fwd_ms / max(fwd_n, 1) / 1e3), 1),
+                      "mcmc_walker_steps_per_s": round(B * S / (mcmc_ms / 1e3), 1),
                       "eloc_only_walkers_per_s": round(B / (lap_ms / max(lap_n, 1) / 1e3), 1)},
     }
 
