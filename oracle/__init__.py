@@ -2,25 +2,78 @@
 
 This package restates, function by function, the reference's MCMC step, log|psi| forward and local
 energy on the CPU (PyTorch CPU tensors, float64 by default; the reference's JAX/haiku/folx stack is not
-installable here).  Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-``bench.py`` may import it, and only as the checker or as the timed CPU baseline.  The product path
-(``deeperwin_amd``) never imports it and fails loudly when its HIP library is missing.
+installable here).  Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU legs of ``bench.py``
+(``cpu_baseline`` and the accuracy sample) may import it, and only as the checker or as the timed CPU
+baseline.  The product path (``deeperwin_amd``) never imports it and fails loudly when its HIP library is
+missing.
 
-Pinning status of each function (what independent check fixes it) is listed in ``PINNING`` below.
+``PINNING`` lists, for each function, the tests that fix it independently of its own tensor code (files
+under ``tests/``).  ``tests/golden`` additionally holds the oracle's outputs on seeded inputs as regression
+fixtures (``test_golden.py``); those detect drift but pin nothing by themselves.
 """
 
 PINNING = {
-    "wavefunction.el_ion_features": "brute-force Python loops (tests/test_oracle_features.py)",
-    "wavefunction.el_el_features": "brute-force Python loops; zero diagonal with zero gradient",
-    "wavefunction.ferminet_embedding": "closed-form constant embedding (zero weights); permutation "
-                                       "equivariance; brute-force per-electron loop",
-    "wavefunction.envelope_orbitals": "brute-force per-element loop; closed-form He-like case",
-    "wavefunction.sum_of_determinants": "Leibniz permutation expansion for n<=4; explicit log-sum-exp",
-    "wavefunction.log_psi_sqr": "antisymmetry under same-spin swaps; joint translation invariance; "
-                                "closed-form He-like psi",
-    "hamiltonian.potential_energy": "brute-force pair loops; analytic H2 / N2 values",
-    "hamiltonian.kinetic_energy": "closed-form He-like E_loc (-k^2 + (k-Z)(1/r1+1/r2) + 1/r12); "
-                                  "central finite differences; reverse-over-reverse Hessian trace",
-    "mcmc.mcmc_steps": "closed form with supplied draws; stationary-distribution test on a 1s density",
-    "evaluation.mean_and_var": "math.fsum",
+    "wavefunction.el_ion_features": [
+        "test_oracle_basics.py::test_el_ion_features_brute_force (per-element loops)",
+        "test_oracle_features.py::test_el_ion_rows_are_ion_major (feature f of ion I at column 4 I + f)",
+    ],
+    "wavefunction.el_el_features": [
+        "test_oracle_basics.py::test_el_el_features_brute_force_and_zero_diagonal (r_j - r_i, zero diagonal "
+        "with zero gradient)",
+    ],
+    "wavefunction.symmetric_features": [
+        "test_oracle_features.py::test_symmetric_features_brute_force (block order, self pair included)",
+        "test_oracle_features.py::test_planted_layout_mistakes_are_caught (swapped spin-mean / pair-mean blocks)",
+    ],
+    "wavefunction.ferminet_embedding": [
+        "test_oracle_features.py::test_slater_matrices_element_by_element (scalar restatement, tests/bruteforce.py)",
+        "test_oracle_physics.py::test_embedding_is_permutation_equivariant_within_spin",
+    ],
+    "wavefunction.envelope_orbitals": [
+        "test_oracle_features.py::test_slater_matrices_element_by_element (n_det >= 2, n_ion >= 2)",
+        "test_oracle_features.py::test_two_det_two_centre_closed_form (column d * n + k = det d, orbital k)",
+        "test_oracle_features.py::test_planted_layout_mistakes_are_caught ((orbital, det) column order)",
+        "test_oracle_physics.py::test_he_like_closed_form_psi_and_local_energy",
+    ],
+    "wavefunction.sum_of_determinants": [
+        "test_oracle_basics.py::test_sum_of_determinants_leibniz",
+        "test_oracle_basics.py::test_log_epsilon_is_inside_the_shifted_log",
+    ],
+    "wavefunction.log_psi_sqr": [
+        "test_oracle_features.py::test_log_psi_sqr_brute_force",
+        "test_oracle_features.py::test_two_det_two_centre_closed_form",
+        "test_oracle_physics.py::test_same_spin_swap_flips_sign_keeps_density",
+        "test_oracle_physics.py::test_joint_translation_invariance",
+        "test_oracle_physics.py::test_he_like_closed_form_psi_and_local_energy",
+    ],
+    "wavefunction.slater_matrices": [
+        "test_oracle_features.py::test_slater_matrices_element_by_element",
+    ],
+    "wavefunction.conditioning": [
+        "not a reference function: a float64 sensitivity estimate used only to size test tolerances",
+    ],
+    "hamiltonian.potential_energy": [
+        "test_oracle_basics.py::test_potential_energy_brute_force_and_h2 (pair loops, analytic H2)",
+        "test_oracle_features.py::test_n2_ion_ion_repulsion (49 / 2.07)",
+    ],
+    "hamiltonian.kinetic_energy": [
+        "test_oracle_physics.py::test_he_like_closed_form_psi_and_local_energy",
+        "test_oracle_features.py::test_two_det_two_centre_closed_form (two dets, two centres)",
+        "test_oracle_physics.py::test_laplacian_central_finite_differences",
+        "test_oracle_physics.py::test_laplacian_matches_reverse_over_reverse_hessian",
+    ],
+    "hamiltonian.local_energy": [
+        "test_oracle_physics.py::test_local_energy_chunking_is_invisible",
+        "test_oracle_features.py::test_two_det_two_centre_closed_form",
+    ],
+    "hamiltonian.mean_and_var": [
+        "test_oracle_physics.py::test_mean_and_var_fsum (math.fsum, NaN dropped, inf kept)",
+    ],
+    "mcmc.mcmc_steps": [
+        "test_oracle_mcmc.py::test_hand_worked_chain (forced accept at the old age, old-rate stepsize update)",
+        "test_oracle_mcmc.py::test_stationary_distribution_hydrogen_1s",
+    ],
+    "mcmc.adjust_stepsize": [
+        "test_oracle_mcmc.py::test_stepsize_clip_and_increase",
+    ],
 }

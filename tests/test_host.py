@@ -66,6 +66,24 @@ def test_el_ion_mapping_fills_every_ion(R, Z, n_up):
         assert abs(up[0] - up[1]) <= 1
 
 
+@pytest.mark.parametrize("R,Z,n_up,expected", [
+    # H2: ties at the first electron go to the lowest (spin, ion) index -> up on ion 0; the second electron
+    # then has (dn, ion 0) at local spin 0 but ion 0 is full, so (dn, ion 1) with max |.| = 1 - e^-1.4
+    ([[0, 0, 0], [1.4, 0, 0]], [1, 1], 1, [0, 1]),
+    # LiH: up/dn alternate on Li (the lowest-index tie, then the zero-spin choice) until Li holds 2 up and
+    # 1 down; the last (down) electron cannot go to the full Li and lands on H
+    ([[0, 0, 0], [3, 0, 0]], [3, 1], 2, [0, 0, 0, 1]),
+    # N2: ion 0 fills first as up, dn, up, dn, up, dn, up (4 up / 3 dn); ion 1 then starts with dn (it
+    # cancels ion 0's +1) and alternates, ending 3 up / 4 dn; up electrons are listed first
+    ([[0, 0, 0], [2.07, 0, 0]], [7, 7], 7, [0, 0, 0, 0, 1, 1, 1, 0, 0, 0, 1, 1, 1, 1]),
+])
+def test_el_ion_mapping_known_answers(R, Z, n_up, expected):
+    """Worked by hand from the greedy rule of configuration.py:1572-1615: each electron takes the
+    (spin, ion) choice with the smallest resulting max |weighted local spin| among ions that are not full
+    and spins with electrons left, ties broken by the lower index (spin-major, then ion)."""
+    assert generate_el_ion_mapping(R, Z, sum(Z), n_up) == expected
+
+
 def test_physical_config_defaults():
     pc = PhysicalConfig(R=[[0, 0, 0], [2.07, 0, 0]], Z=[7, 7])
     assert pc.n_electrons == 14 and pc.n_up == 7 and len(pc.el_ion_mapping) == 14

@@ -168,3 +168,25 @@ def test_mean_and_var_fsum():
     m, v = oh.mean_and_var(e)
     assert abs(m - math.fsum([1.0, 2.0, 4.0]) / 3) < 1e-15
     assert abs(v - math.fsum([(x - m) ** 2 for x in (1.0, 2.0, 4.0)]) / 3) < 1e-15
+    # nanmean drops NaN only: an infinite local energy makes the mean infinite (evaluation.py:148-150)
+    m, v = oh.mean_and_var(np.array([1.0, np.inf, np.nan]))
+    assert m == np.inf
+
+
+def test_pinning_table_names_existing_tests():
+    import ast
+    import os
+
+    import oracle
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    for fn, pins in oracle.PINNING.items():
+        for pin in pins:
+            if "::" not in pin:
+                assert pin.startswith("not a reference function"), (fn, pin)
+                continue
+            fname, rest = pin.split("::", 1)
+            test = rest.split(" ")[0]
+            tree = ast.parse(open(os.path.join(here, fname)).read())
+            names = {n.name for n in ast.walk(tree) if isinstance(n, ast.FunctionDef)}
+            assert test in names, (fn, pin)
