@@ -53,6 +53,7 @@ SIGNATURES = {
     "dw_local_energy": ([_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
     "dw_mcmc_steps": ([_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _f32, _f32, _f32, _vp,
                        _vp, _vp], ctypes.c_int),
+    "dw_energy_moments": ([_vp, _i32, _vp, _vp], ctypes.c_int),
     "dw_workspace_size": ([_vp, _i32, _i32], ctypes.c_size_t),
     "dw_model_set_chunk": ([_vp, _i32], ctypes.c_int),
     "dw_model_set_workspace": ([_vp, _vp, ctypes.c_size_t], ctypes.c_int),
@@ -131,9 +132,26 @@ class NativeModel:
                 pass
             self.handle = None
 
+    @staticmethod
+    def _leaf_token(v):
+        # numpy arrays are compared by content (they may be mutated in place); torch tensors by identity and
+        # their in-place version counter
+        if hasattr(v, "_version"):
+            return ("t", v._version)
+        a = np.asarray(v)
+        return ("a", a.shape, a.dtype.str, hash(np.ascontiguousarray(a).tobytes()))
+
     def set_params(self, params: Dict) -> None:
-        token = tuple(id(v) for leaves in params.values() for v in leaves.values())
-        if token == self._param_token:
+        """Upload the leaves of `params` unless they are the very objects of the last upload, unchanged.
+
+        The model keeps references to the uploaded leaves, so an identity match cannot come from a freed
+        and reused object; torch tensors also carry their version counter (in-place updates) and numpy
+        arrays a content hash.  `invalidate()` forces the next call to upload."""
+        leaves_now = [(p, k, v) for p, leaves in params.items() for k, v in leaves.items()]
+        last = self._param_token
+        if last is not None and len(last) == len(leaves_now) and all(
+                p == lp and k == lk and v is lv and self._leaf_token(v) == lt
+                for (p, k, v), (lp, lk, lv, lt) in zip(leaves_now, last)):
             return
         for path, leaves in params.items():
             for leaf, val in leaves.items():
@@ -144,7 +162,11 @@ class NativeModel:
                                                    arr.ctypes.data_as(ctypes.c_void_p), shp, arr.ndim))
         if not self.lib.dw_model_ready(self.handle):
             raise DwError("not every parameter leaf of the wavefunction was provided")
-        self._param_token = token
+        self._param_token = [(p, k, v, self._leaf_token(v)) for p, k, v in leaves_now]
+
+    def invalidate(self) -> None:
+        """Forget the last upload: the next set_params uploads every leaf."""
+        self._param_token = None
 
     def set_tuning(self, **choices) -> None:
         """Launch-shape choices (fields of dw_tuning); unnamed fields keep their current value."""
@@ -211,3 +233,15 @@ class NativeModel:
                                       _ptr(normals), _ptr(uniforms), n_steps, _ptr(chain), int(max_age),
                                       int(interval), float(target), float(lo), float(hi), _ptr(out_acc),
                                       _ptr(out_flags), _stream()))
+
+
+def energy_moments(e_loc):
+    """(E_mean, E_var) of a CUDA float32 tensor of local energies as a CUDA float64 tensor of 2 (device
+    side, ordered on the current stream): nanmean and nanmean of squared deviations
+    (optimization/evaluation.py:143-151)."""
+    import torch
+
+    out = torch.empty(2, device=e_loc.device, dtype=torch.float64)
+    e = e_loc.reshape(-1).to(torch.float32).contiguous()
+    _check(load_library().dw_energy_moments(_ptr(e), int(e.numel()), _ptr(out), _stream()))
+    return out

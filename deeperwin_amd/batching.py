@@ -146,10 +146,12 @@ class GeometryBatch:
         return [(gm.r.cpu().numpy(), gm.lp.cpu().numpy(), gm.age.cpu().numpy(), gm.e.cpu().numpy())
                 for gm in self.geoms]
 
-    def flops_per_unit(self) -> float:
-        """Algorithmic FLOPs of one unit (the scope table's per-walker count: n_steps forwards + one
-        Laplacian pass), summed over geometries."""
-        return float(sum(self.walkers * unit_flops(gm.shape, self.n_steps) for gm in self.geoms))
+    def flops_per_unit(self, executed: bool = False) -> float:
+        """FLOPs of one unit (n_steps forwards + one Laplacian pass per walker), summed over geometries: the
+        scope table's count of the reference's layer shapes, or with `executed` the count of the algorithm
+        this library runs (spin means multiplied once per walker)."""
+        f = executed_unit_flops if executed else unit_flops
+        return float(sum(self.walkers * f(gm.shape, self.n_steps) for gm in self.geoms))
 
 
 def forward_flops(shape: ModelShape) -> float:
@@ -170,3 +172,43 @@ def laplacian_flops(shape: ModelShape) -> float:
 
 def unit_flops(shape: ModelShape, n_steps: int) -> float:
     return n_steps * forward_flops(shape) + laplacian_flops(shape)
+
+
+# ---- FLOPs of the algorithm as this library runs it ----
+# The counts above follow the reference's layer shapes: every electron multiplies the full symmetric
+# feature vector f = [h, mean_up h, mean_dn h, pair_up, pair_dn] (width 3 d + 2 e) by W.  The library
+# splits W by rows: the per-electron part [h | pair_up | pair_dn] (width d + 2 e) is one GEMM per electron
+# and the spin means [mean_up h | mean_dn h] (width 2 d) are multiplied once per walker, so fewer FLOPs run
+# for the same linear map.  The Laplacian pass carries C = 3 n + 2 channels (value, 3 n tangents,
+# Laplacian) through the 1-el, spin-mean and orbital GEMMs and 5 channels (value, 3 tangents along
+# r_j - r_i, Laplacian) through the 2-el stream.
+
+def _layer_widths(shape: ModelShape):
+    I, W1, W2, L = shape.n_ion, shape.width1, shape.width2, shape.n_iter
+    for l in range(L):
+        kh = 4 * I if l == 0 else W1
+        ke = 4 if l == 0 else W2
+        yield kh, ke
+
+
+def executed_forward_flops(shape: ModelShape) -> float:
+    """One value pass per walker: 2 n (d + 2 e) W1 per electron and 2 (2 d) W1 per walker per layer, the 2-el
+    stream, the orbital GEMM and the LU."""
+    n, D, W1, W2, L = shape.n_el, shape.n_det, shape.width1, shape.width2, shape.n_iter
+    one = sum(2 * n * (kh + 2 * ke) * W1 + 2 * (2 * kh) * W1 for kh, ke in _layer_widths(shape))
+    two = 2 * n * n * (4 * W2 + (L - 2) * W2 * W2)
+    return one + two + 2 * n * W1 * D * n + D * (2.0 / 3.0) * n ** 3
+
+
+def executed_laplacian_flops(shape: ModelShape) -> float:
+    """One Laplacian pass per walker: C = 3 n + 2 channels through the 1-el, spin-mean and orbital GEMMs,
+    5 channels through the 2-el stream, and the determinant stage of the scope table."""
+    n, D, W1, W2, L = shape.n_el, shape.n_det, shape.width1, shape.width2, shape.n_iter
+    C = 3 * n + 2
+    one = sum(2 * n * C * (kh + 2 * ke) * W1 + 2 * C * (2 * kh) * W1 for kh, ke in _layer_widths(shape))
+    two = 5 * 2 * n * n * (4 * W2 + (L - 2) * W2 * W2)
+    return one + two + C * 2 * n * W1 * D * n + D * 2 * n ** 3 + D * 3 * n * (2 * n ** 3 + 4 * n ** 2)
+
+
+def executed_unit_flops(shape: ModelShape, n_steps: int) -> float:
+    return n_steps * executed_forward_flops(shape) + executed_laplacian_flops(shape)

@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "api_internal.h"
 #include "dwmi.h"
 #include "kernels.h"
 
@@ -36,20 +37,8 @@ dw_status fail(dw_status code, const std::string& msg) {
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
-  bool ext = false;  // points into a caller-owned workspace
-  void assign(void* ptr, size_t n) {
-    if (p && !ext) (void)hipFree(p);
-    p = ptr;
-    bytes = n;
-    ext = true;
-  }
   hipError_t ensure(size_t need) {
     if (need <= bytes) return hipSuccess;
-    if (ext) {  // leave the caller's region; fall back to an own allocation
-      p = nullptr;
-      bytes = 0;
-      ext = false;
-    }
     if (p) {
       hipError_t e = hipFree(p);
       if (e != hipSuccess) return e;
@@ -61,10 +50,9 @@ struct DevBuf {
     return e;
   }
   void release() {
-    if (p && !ext) (void)hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
-    ext = false;
   }
   template <typename T>
   T* as() const {
@@ -130,13 +118,26 @@ struct Weights {
   }
 };
 
-// Activation workspace for one activation type.
-struct Workspace {
-  std::vector<DevBuf> X;
-  DevBuf MU, V, Z, H, O, A;
-  void release() {
-    for (auto& b : X) b.release();
-    for (DevBuf* b : {&MU, &V, &Z, &H, &O, &A}) b->release();
+// Device scratch of one wavefunction pass over B walkers with C channels (activation type T): the
+// activation buffers of every layer and the per-determinant results.  All of it is carved, 256-byte
+// aligned, out of one region: the model's own arena, a caller-owned workspace, or a fixed scratch array.
+template <typename T>
+struct PassBufs {
+  T* X[4] = {nullptr, nullptr, nullptr, nullptr};
+  T *MU = nullptr, *V = nullptr, *Z = nullptr, *H = nullptr, *O = nullptr, *A = nullptr;
+  double *ld = nullptr, *sd = nullptr, *g = nullptr, *lap = nullptr;
+  float* dcond = nullptr;
+};
+
+// Bump allocator over a region; with base == nullptr it only counts bytes.
+struct Carver {
+  char* base = nullptr;
+  size_t off = 0;
+  template <typename U>
+  U* take(size_t count) {
+    U* q = base ? (U*)(base + off) : nullptr;
+    off += (count * sizeof(U) + 255) & ~(size_t)255;
+    return q;
   }
 };
 
@@ -155,8 +156,9 @@ struct dw_model {
   size_t ext_ws_bytes = 0;
   GemmPrec prec = GEMM_F32;
   Weights wf, wd;    // float and double layouts
-  Workspace sf, sd;  // float and double activations
-  DevBuf ldd, sdd, gd, lapd, dcond, kappa_w;
+  DevBuf arena_f, arena_d;  // pass scratch (PassBufs) of the float and double activation types
+  DevBuf kappa_w;           // per-walker conditioning of the last pass (dw_model_conditioning)
+  long kappa_n = 0;         // walkers covered by kappa_w
   DevBuf rprop, lpprop, cnt;
   DevBuf sub_r, sub_e, sub_lp, sub_idx, own_flags;  // FP64 recompute of flagged walkers
   // measurement hooks
@@ -176,9 +178,7 @@ struct dw_model {
     for (auto e : event_pool) (void)hipEventDestroy(e);
     wf.release();
     wd.release();
-    sf.release();
-    sd.release();
-    for (DevBuf* b : {&ldd, &sdd, &gd, &lapd, &dcond, &kappa_w, &rprop, &lpprop, &cnt, &sub_r, &sub_e, &sub_lp,
+    for (DevBuf* b : {&arena_f, &arena_d, &kappa_w, &rprop, &lpprop, &cnt, &sub_r, &sub_e, &sub_lp,
                       &sub_idx, &own_flags})
       b->release();
   }
@@ -314,8 +314,12 @@ dw_status prepare_weights(dw_model* m, Weights& W, bool dbl) {
   return DW_OK;
 }
 
-dw_status prepare(dw_model* m, bool need_double) {
+// Device weights for the model's precision (and the double layout when need_double).  A re-upload
+// overwrites buffers that kernels queued earlier on `s` may still read, so `s` is drained first; the
+// uploads themselves are complete when prepare_weights returns.
+dw_status prepare(dw_model* m, bool need_double, hipStream_t s) {
   if (m->dirty) {
+    if (m->wf.ready || m->wd.ready) HIP_TRY(hipStreamSynchronize(s));
     m->wf.ready = false;
     m->wd.ready = false;
     m->dirty = false;
@@ -338,67 +342,51 @@ dw_status prepare(dw_model* m, bool need_double) {
   return DW_OK;
 }
 
-size_t ws_bytes(const dw_model* m, long B, int C, size_t esz) {
+// Lay out the scratch of one pass of B walkers with C channels at `base` (nullptr: count only) and
+// return its size in bytes.
+template <typename T>
+size_t layout_pass(const dw_model* m, long B, int C, char* base, PassBufs<T>* out) {
   const int Cp = rows_per_electron(C);
   const long rows = B * m->n * Cp;
-  size_t tot = 0;
-  int kmmax = 0;
-  for (int l = 0; l < m->L; ++l) {
-    tot += (size_t)rows * m->Kx[l] * esz;
-    kmmax = std::max(kmmax, m->Km[l]);
-  }
-  tot += (size_t)B * Cp * kmmax * esz + (size_t)B * Cp * m->W1 * esz * (C == 1 ? 4 : 1);
-  if (esz == 8 && C > 1) tot += (size_t)rows * m->W1 * esz;  // unfused pre-activations (FP64 form)
-  tot += (size_t)rows * m->W1 * esz + (size_t)rows * m->NO * esz;
-  if (!det_reads_orbitals(m->n, C)) tot += (size_t)B * m->D * C * m->n * m->n * esz;  // envelope x orbitals
-  tot += (size_t)B * m->D * (8 * 2 + 4);
-  if (C > 1) tot += (size_t)B * m->D * (3 * m->n + 1) * 8;
-  return tot;
-}
-
-// Size the workspace for B walkers with C channels and activation size esz.
-dw_status ensure_ws(dw_model* m, Workspace& w, long B, int C, size_t esz) {
-  const int Cp = rows_per_electron(C);
-  const long rows = B * m->n * Cp;
-  w.X.resize(m->L);
-  if (m->ext_ws) {  // carve the activation buffers out of the caller's workspace (256-byte aligned)
-    size_t off = 0;
-    bool fits = true;
-    auto carve = [&](DevBuf& b, size_t n) {
-      if (off + n > m->ext_ws_bytes) {
-        fits = false;
-        return;
-      }
-      b.assign((char*)m->ext_ws + off, n);
-      off += (n + 255) & ~(size_t)255;
-    };
-    int kmmax = 0;
-    for (int l = 0; l < m->L; ++l) kmmax = std::max(kmmax, m->Km[l]);
-    for (int l = 0; l < m->L; ++l) carve(w.X[l], (size_t)rows * m->Kx[l] * esz);
-    carve(w.MU, (size_t)B * Cp * kmmax * esz);
-    carve(w.V, (size_t)B * Cp * m->W1 * esz * (C == 1 ? 4 : 1));
-    if (esz == 8 && C > 1) carve(w.Z, (size_t)rows * m->W1 * esz);
-    carve(w.H, (size_t)rows * m->W1 * esz);
-    carve(w.O, (size_t)rows * m->NO * esz);
-    if (!det_reads_orbitals(m->n, C)) carve(w.A, (size_t)B * m->D * C * m->n * m->n * esz);
-    if (!fits)
-      return fail(DW_ERR_INVALID, "the workspace given to dw_model_set_workspace is smaller than dw_workspace_size");
-  }
-  for (int l = 0; l < m->L; ++l) HIP_TRY(w.X[l].ensure((size_t)rows * m->Kx[l] * esz));
   int kmmax = 0;
   for (int l = 0; l < m->L; ++l) kmmax = std::max(kmmax, m->Km[l]);
-  HIP_TRY(w.MU.ensure((size_t)B * Cp * kmmax * esz));
-  HIP_TRY(w.V.ensure((size_t)B * Cp * m->W1 * esz * (C == 1 ? 4 : 1)));  // value pass: up to 4 K-split parts
-  if (esz == 8 && C > 1) HIP_TRY(w.Z.ensure((size_t)rows * m->W1 * esz));
-  HIP_TRY(w.H.ensure((size_t)rows * m->W1 * esz));
-  HIP_TRY(w.O.ensure((size_t)rows * m->NO * esz));
-  if (!det_reads_orbitals(m->n, C)) HIP_TRY(w.A.ensure((size_t)B * m->D * C * m->n * m->n * esz));
-  HIP_TRY(m->ldd.ensure((size_t)B * m->D * 8));
-  HIP_TRY(m->sdd.ensure((size_t)B * m->D * 8));
-  HIP_TRY(m->dcond.ensure((size_t)B * m->D * 4));
-  HIP_TRY(m->gd.ensure((size_t)std::max<long>(1, C > 1 ? B * m->D * 3 * m->n : 1) * 8));
-  HIP_TRY(m->lapd.ensure((size_t)std::max<long>(1, C > 1 ? B * m->D : 1) * 8));
-  HIP_TRY(m->kappa_w.ensure((size_t)B * 4));
+  Carver cv{base};
+  PassBufs<T> b;
+  for (int l = 0; l < m->L; ++l) b.X[l] = cv.take<T>((size_t)rows * m->Kx[l]);
+  b.MU = cv.take<T>((size_t)B * Cp * kmmax);
+  b.V = cv.take<T>((size_t)B * Cp * m->W1 * (C == 1 ? 4 : 1));  // value pass: up to 4 K-split parts
+  if (sizeof(T) == 8 && C > 1) b.Z = cv.take<T>((size_t)rows * m->W1);  // unfused pre-activations (FP64 form)
+  b.H = cv.take<T>((size_t)rows * m->W1);
+  b.O = cv.take<T>((size_t)rows * m->NO);
+  if (!det_reads_orbitals(m->n, C)) b.A = cv.take<T>((size_t)B * m->D * C * m->n * m->n);  // envelope x orbitals
+  b.ld = cv.take<double>((size_t)B * m->D);
+  b.sd = cv.take<double>((size_t)B * m->D);
+  b.dcond = cv.take<float>((size_t)B * m->D);
+  b.g = cv.take<double>((size_t)std::max<long>(1, C > 1 ? B * m->D * 3 * m->n : 1));
+  b.lap = cv.take<double>((size_t)std::max<long>(1, C > 1 ? B * m->D : 1));
+  if (out) *out = b;
+  return cv.off;
+}
+
+size_t pass_bytes(const dw_model* m, long B, int C, bool dbl) {
+  return dbl ? layout_pass<double>(m, B, C, nullptr, nullptr) : layout_pass<float>(m, B, C, nullptr, nullptr);
+}
+
+// Scratch of one pass from the caller-owned workspace, else from the model's arena (grown on demand).
+template <typename T>
+dw_status pass_scratch(dw_model* m, long B, int C, PassBufs<T>* pb) {
+  const size_t need = layout_pass<T>(m, B, C, nullptr, nullptr);
+  char* base;
+  if (m->ext_ws) {
+    if (need > m->ext_ws_bytes)
+      return fail(DW_ERR_INVALID, "the workspace given to dw_model_set_workspace is smaller than dw_workspace_size");
+    base = (char*)m->ext_ws;
+  } else {
+    DevBuf& a = sizeof(T) == 8 ? m->arena_d : m->arena_f;
+    HIP_TRY(a.ensure(need));
+    base = a.as<char>();
+  }
+  layout_pass<T>(m, B, C, base, pb);
   return DW_OK;
 }
 
@@ -445,25 +433,23 @@ void gemm_t(const dw_model*, const double* A, int lda, const double* Bt, const D
 }
 
 // One pass of the wavefunction over B walkers with C channels (C == 1: value; C == 3n+2: Laplacian).
+// `ws` is the pass's scratch (layout_pass); out_kappa (may be null) receives the per-walker conditioning.
 template <typename T>
-dw_status run_pass(dw_model* m, const float* r, long B, const float* R, const float* Z, int C, hipStream_t s,
-                   float* out_lp, float* out_phase, float* out_eloc, uint32_t* out_flags, float cond_thr) {
+dw_status run_pass(dw_model* m, const PassBufs<T>& ws, const float* r, long B, const float* R, const float* Z, int C,
+                   hipStream_t s, float* out_lp, float* out_phase, float* out_eloc, uint32_t* out_flags,
+                   float* out_kappa, float cond_thr) {
   constexpr bool DBL = sizeof(T) == 8;
   Weights& W = DBL ? m->wd : m->wf;
-  Workspace& ws = DBL ? m->sd : m->sf;
-  dw_status st = ensure_ws(m, ws, B, C, sizeof(T));
-  if (st) return st;
   const int Cp = rows_per_electron(C);
   Shape sh{(int)B, m->n, m->u, m->I, m->D, C, Cp, m->W1, m->W2, m->L};
   const long rows = B * m->n * Cp;
   Timed pass_timer(m, C > 1 ? DW_TIMER_LOCAL_ENERGY : DW_TIMER_FORWARD, s);
-  std::vector<T*> X(m->L);
-  for (int l = 0; l < m->L; ++l) X[l] = ws.X[l].as<T>();
+  T* const* X = ws.X;
   launch_features0<T>(r, R, X[0], m->Kx[0], sh, s);
-  launch_twoel<T>(r, X.data(), m->Kx.data(), m->Kh.data(), W.W2e.as<T>(), W.b2e.as<T>(), sh, s);
+  launch_twoel<T>(r, X, m->Kx.data(), m->Kh.data(), W.W2e.as<T>(), W.b2e.as<T>(), sh, s);
   const long vstride = B * Cp * (long)m->W1;
   for (int l = 0; l < m->L; ++l) {
-    launch_spin_mean<T>(X[l], m->Kx[l], m->Kh[l], ws.MU.as<T>(), m->Km[l], sh, s);
+    launch_spin_mean<T>(X[l], m->Kx[l], m->Kh[l], ws.MU, m->Km[l], sh, s);
     // value pass: split K of the per-walker spin-mean GEMM so that it fills the chip (its M is only B)
     int vs = 1;
     if (!DBL && C == 1 && m->prec == GEMM_F32) {
@@ -482,13 +468,13 @@ dw_status run_pass(dw_model* m, const float* r, long B, const float* R, const fl
     ev.ksplit = vs;
     ev.cstride = vstride;
     ev.tile = C == 1 ? -1 : m->tune.lap_mu_tile;
-    gemm_t(m, ws.MU.as<T>(), m->Km[l], W.Wmu[l].as<T>(), W.Wmu_hi[l], W.Wmu_lo[l], m->Km[l], ws.V.as<T>(), m->W1,
+    gemm_t(m, ws.MU, m->Km[l], W.Wmu[l].as<T>(), W.Wmu_hi[l], W.Wmu_lo[l], m->Km[l], ws.V, m->W1,
            B * Cp, m->W1, m->Km[l], ev, s);
     const bool last = (l == m->L - 1);
-    T* outp = last ? ws.H.as<T>() : X[l + 1];
+    T* outp = last ? ws.H : X[l + 1];
     const int ldo = last ? m->W1 : m->Kx[l + 1];
     GemmEpiT<T> em;
-    em.V = ws.V.as<T>();
+    em.V = ws.V;
     em.ldv = m->W1;
     em.C = Cp;
     em.nC = m->n * Cp;
@@ -503,7 +489,7 @@ dw_status run_pass(dw_model* m, const float* r, long B, const float* R, const fl
       em.permute = last ? 1 : 0;
       if constexpr (!DBL) {
         if (m->prec == GEMM_F32 && m->tune.fwd_main_tile == 3) {  // 16x16-MFMA tiles of 96 electrons
-          gemm_value16(X[l], m->Kx[l], W.Wmain[l].as<float>(), m->Kx[l], ws.V.as<float>(), m->W1, vs, vstride, outp,
+          gemm_value16(X[l], m->Kx[l], W.Wmain[l].as<float>(), m->Kx[l], ws.V, m->W1, vs, vstride, outp,
                        ldo, last ? 1 : 0, sh, s);
           continue;
         }
@@ -513,12 +499,12 @@ dw_status run_pass(dw_model* m, const float* r, long B, const float* R, const fl
     } else {
       Timed t(m, l >= 1 ? DW_TIMER_GEMM_1EL_LAP : -1, s);
       if constexpr (DBL) {
-        gemm_t(m, X[l], m->Kx[l], W.Wmain[l].as<T>(), W.Wmain_hi[l], W.Wmain_lo[l], m->Kx[l], ws.Z.as<T>(), m->W1,
+        gemm_t(m, X[l], m->Kx[l], W.Wmain[l].as<T>(), W.Wmain_hi[l], W.Wmain_lo[l], m->Kx[l], ws.Z, m->W1,
                rows, m->W1, m->Kx[l], em, s);
-        launch_tanh_rule<T>(ws.Z.as<T>(), outp, ldo, last ? 1 : 0, sh, s);
+        launch_tanh_rule<T>(ws.Z, outp, ldo, last ? 1 : 0, sh, s);
       } else {
         gemm_lap(m->prec, X[l], m->Kx[l], W.Wmain[l].as<float>(), W.Wmain_hi[l].as<uint16_t>(),
-                 W.Wmain_lo[l].as<uint16_t>(), m->Kx[l], ws.V.as<float>(), m->W1, outp, ldo, last ? 1 : 0, sh,
+                 W.Wmain_lo[l].as<uint16_t>(), m->Kx[l], ws.V, m->W1, outp, ldo, last ? 1 : 0, sh,
                  m->tune.lap_electrons, m->tune.lap_cols, s);
       }
     }
@@ -527,32 +513,120 @@ dw_status run_pass(dw_model* m, const float* r, long B, const float* R, const fl
   const long up_rows = B * m->u * Cp, dn_rows = B * m->dn * Cp;
   GemmEpiT<T> e0;
   e0.tile = C == 1 ? m->tune.fwd_orb_tile : m->tune.lap_orb_tile;
-  gemm_t(m, ws.H.as<T>(), m->W1, W.Wo[0].as<T>(), W.Wo_hi[0], W.Wo_lo[0], m->W1, ws.O.as<T>(), m->NO, up_rows, m->NO,
+  gemm_t(m, ws.H, m->W1, W.Wo[0].as<T>(), W.Wo_hi[0], W.Wo_lo[0], m->W1, ws.O, m->NO, up_rows, m->NO,
          m->W1, e0, s);
-  gemm_t(m, ws.H.as<T>() + up_rows * m->W1, m->W1, W.Wo[1].as<T>(), W.Wo_hi[1], W.Wo_lo[1], m->W1,
-         ws.O.as<T>() + up_rows * m->NO, m->NO, dn_rows, m->NO, m->W1, e0, s);
-  const OrbSrc<T> src{ws.O.as<T>(), m->NO, r, R, W.kappa.as<T>(), W.wenv.as<T>()};
+  gemm_t(m, ws.H + up_rows * m->W1, m->W1, W.Wo[1].as<T>(), W.Wo_hi[1], W.Wo_lo[1], m->W1,
+         ws.O + up_rows * m->NO, m->NO, dn_rows, m->NO, m->W1, e0, s);
+  const OrbSrc<T> src{ws.O, m->NO, r, R, W.kappa.as<T>(), W.wenv.as<T>()};
   if (!det_reads_orbitals(m->n, C))
-    launch_envelope<T>(ws.O.as<T>(), m->NO, r, R, W.kappa.as<T>(), W.wenv.as<T>(), ws.A.as<T>(), sh, s);
+    launch_envelope<T>(ws.O, m->NO, r, R, W.kappa.as<T>(), W.wenv.as<T>(), ws.A, sh, s);
   {
     Timed t(m, C > 1 ? DW_TIMER_DET_LAP : -1, s);
-    launch_det<T>(ws.A.as<T>(), src, m->ldd.as<double>(), m->sdd.as<double>(), m->gd.as<double>(),
-                  m->lapd.as<double>(), m->dcond.as<float>(), sh, s);
+    launch_det<T>(ws.A, src, ws.ld, ws.sd, ws.g, ws.lap, ws.dcond, sh, s);
   }
-  launch_combine(m->ldd.as<double>(), m->sdd.as<double>(), m->gd.as<double>(), m->lapd.as<double>(),
-                 m->dcond.as<float>(), cond_thr, r, R, Z, out_lp, out_phase, out_eloc, out_flags,
-                 m->kappa_w.as<float>(), sh, s);
+  launch_combine(ws.ld, ws.sd, ws.g, ws.lap, ws.dcond, cond_thr, r, R, Z, out_lp, out_phase, out_eloc, out_flags,
+                 out_kappa, sh, s);
   HIP_TRY(hipGetLastError());
   return DW_OK;
 }
 
+// One pass in the model's precision, its scratch from the model (arena or caller-owned workspace).
 dw_status pass(dw_model* m, const float* r, long B, const float* R, const float* Z, int C, hipStream_t s,
-               float* out_lp, float* out_phase, float* out_eloc, uint32_t* out_flags, float cond_thr) {
-  if (m->prec == GEMM_F64) return run_pass<double>(m, r, B, R, Z, C, s, out_lp, out_phase, out_eloc, out_flags, 0.f);
-  return run_pass<float>(m, r, B, R, Z, C, s, out_lp, out_phase, out_eloc, out_flags, cond_thr);
+               float* out_lp, float* out_phase, float* out_eloc, uint32_t* out_flags, float* out_kappa,
+               float cond_thr) {
+  if (m->prec == GEMM_F64) {
+    PassBufs<double> pb;
+    dw_status st = pass_scratch(m, B, C, &pb);
+    if (st) return st;
+    return run_pass<double>(m, pb, r, B, R, Z, C, s, out_lp, out_phase, out_eloc, out_flags, out_kappa, 0.f);
+  }
+  PassBufs<float> pb;
+  dw_status st = pass_scratch(m, B, C, &pb);
+  if (st) return st;
+  return run_pass<float>(m, pb, r, B, R, Z, C, s, out_lp, out_phase, out_eloc, out_flags, out_kappa, cond_thr);
 }
 
 }  // namespace
+
+namespace dwmi {
+
+dw_status set_error(dw_status code, const std::string& msg) { return fail(code, msg); }
+
+dw_status mcmc_run(dw_model* m, const McmcArgs& a, const FixedScratch* fx, hipStream_t s) {
+  if (!m || !a.r || !a.R || !a.Z || !a.log_psi_sqr || !a.walker_age || !a.chain)
+    return fail(DW_ERR_INVALID, "null argument");
+  if (a.n_walkers <= 0 || a.n_steps < 0) return fail(DW_ERR_INVALID, "bad sizes");
+  if (a.n_steps > 0 && (!a.normals || !a.uniforms)) return fail(DW_ERR_INVALID, "draws required");
+  const bool dbl = m->prec == GEMM_F64;
+  const long B = a.n_walkers;
+  const int cnt3 = (int)(B * m->n * 3);
+  float *rprop, *lpprop;
+  int32_t* cnt;
+  PassBufs<float> pf;
+  PassBufs<double> pd;
+  if (fx) {
+    char* const base = fx->check_only ? nullptr : fx->base;
+    const size_t scratch_bytes = fx->bytes;
+    Carver cv{base, pass_bytes(m, B, 1, dbl)};  // the pass's layout first, then the chain's own buffers
+    rprop = cv.take<float>((size_t)cnt3);
+    lpprop = cv.take<float>((size_t)B);
+    cnt = cv.take<int32_t>((size_t)std::max(a.n_steps, 1));
+    if (cv.off > scratch_bytes) {
+      const size_t per_walker = (pass_bytes(m, 64, 1, dbl) + 64 * (size_t)(m->n * 3 + 1) * 4) / 64;
+      return fail(DW_ERR_INVALID, "dw_mcmc_steps_noalloc: " + std::to_string(B) + " walkers need " +
+                                      std::to_string(cv.off) + " bytes of scratch, more than its " +
+                                      std::to_string(scratch_bytes) + " (about " +
+                                      std::to_string(scratch_bytes / std::max<size_t>(per_walker, 1)) +
+                                      " walkers of this model fit)");
+    }
+    if (m->dirty || !(dbl ? m->wd.ready : m->wf.ready))
+      return fail(DW_ERR_STATE, "dw_mcmc_steps_noalloc: the weights are not on the device; call dw_logpsi, "
+                                "dw_mcmc_steps or dw_local_energy once after the last dw_model_set_param");
+    if (fx->check_only) return DW_OK;
+    if (dbl)
+      layout_pass<double>(m, B, 1, base, &pd);
+    else
+      layout_pass<float>(m, B, 1, base, &pf);
+  } else {
+    dw_status st = prepare(m, false, s);
+    if (st) return st;
+    HIP_TRY(m->rprop.ensure((size_t)cnt3 * 4));
+    HIP_TRY(m->lpprop.ensure((size_t)B * 4));
+    HIP_TRY(m->cnt.ensure((size_t)std::max(a.n_steps, 1) * 4));
+    rprop = m->rprop.as<float>();
+    lpprop = m->lpprop.as<float>();
+    cnt = m->cnt.as<int32_t>();
+    st = dbl ? pass_scratch(m, B, 1, &pd) : pass_scratch(m, B, 1, &pf);
+    if (st) return st;
+  }
+  // the timing hooks create events; the non-allocating variant records none
+  const bool timing = m->timing;
+  if (fx) m->timing = false;
+  auto value_pass = [&](const float* rin, float* lp_out) {
+    return dbl ? run_pass<double>(m, pd, rin, B, a.R, a.Z, 1, s, lp_out, nullptr, nullptr, nullptr, nullptr, 0.f)
+               : run_pass<float>(m, pf, rin, B, a.R, a.Z, 1, s, lp_out, nullptr, nullptr, nullptr, nullptr, 0.f);
+  };
+  // log_psi_sqr is re-evaluated on entry (mcmc.py:396)
+  dw_status st = value_pass(a.r, a.log_psi_sqr);
+  if (!st && a.n_steps > 0) {
+    launch_fill_i32(cnt, 0, a.n_steps, s);
+    for (int k = 0; k < a.n_steps && !st; ++k) {
+      launch_propose(a.r, a.normals + (size_t)k * cnt3, a.chain, rprop, cnt3, s);
+      st = value_pass(rprop, lpprop);
+      if (st) break;
+      launch_accept(a.r, rprop, a.log_psi_sqr, lpprop, a.uniforms + (size_t)k * B, a.walker_age, a.max_age, cnt + k,
+                    a.out_flags, (int)B, m->n, s);
+      launch_chain_update(a.chain, cnt + k, (int)B, a.out_acc ? a.out_acc + k : nullptr, a.stepsize_update_interval,
+                          a.target_acceptance, a.min_stepsize, a.max_stepsize, s);
+    }
+  }
+  m->timing = timing;
+  if (st) return st;
+  HIP_TRY(hipGetLastError());
+  return DW_OK;
+}
+
+}  // namespace dwmi
 
 extern "C" {
 
@@ -704,8 +778,8 @@ dw_status dw_model_get_tuning(const dw_model* m, dw_tuning* out) {
 dw_status dw_model_set_workspace(dw_model* m, void* device_ptr, size_t bytes) {
   if (!m) return fail(DW_ERR_INVALID, "null model");
   if (device_ptr && ((uintptr_t)device_ptr & 255)) return fail(DW_ERR_INVALID, "workspace must be 256-byte aligned");
-  m->sf.release();
-  m->sd.release();
+  m->arena_f.release();
+  m->arena_d.release();
   m->ext_ws = device_ptr;
   m->ext_ws_bytes = device_ptr ? bytes : 0;
   return DW_OK;
@@ -719,15 +793,22 @@ dw_status dw_model_set_chunk(dw_model* m, int32_t max_walkers_per_pass) {
 
 size_t dw_workspace_size(const dw_model* m, int32_t n_walkers, int32_t mode) {
   if (!m || n_walkers <= 0) return 0;
-  const int C = mode ? 3 * m->n + 2 : 1;
+  const bool dbl = m->prec == GEMM_F64;
+  if (!mode) return pass_bytes(m, n_walkers, 1, dbl);
+  const int C = 3 * m->n + 2;
   long B = n_walkers;
-  if (mode && m->chunk > 0 && B > m->chunk) B = m->chunk;
-  return ws_bytes(m, B, C, m->prec == GEMM_F64 ? 8 : 4);
+  if (m->chunk > 0 && B > m->chunk) B = m->chunk;
+  size_t bytes = pass_bytes(m, B, C, dbl);
+  // the FP64 recompute of flagged walkers carves double activations from the same region
+  if (!dbl && m->d.fp64_cond_threshold > 0.f) bytes = std::max(bytes, pass_bytes(m, B, C, true));
+  return bytes;
 }
 
 dw_status dw_model_conditioning(dw_model* m, float* out, int32_t n_walkers, dw_stream stream) {
   if (!m || !out || n_walkers <= 0) return fail(DW_ERR_INVALID, "bad argument");
-  if ((size_t)n_walkers * 4 > m->kappa_w.bytes) return fail(DW_ERR_STATE, "no pass of that size has run");
+  if (n_walkers > m->kappa_n)
+    return fail(DW_ERR_STATE, "the last dw_logpsi / dw_local_energy call covered " + std::to_string(m->kappa_n) +
+                                  " walkers");
   HIP_TRY(hipMemcpyAsync(out, m->kappa_w.p, (size_t)n_walkers * 4, hipMemcpyDeviceToDevice, (hipStream_t)stream));
   return DW_OK;
 }
@@ -736,9 +817,12 @@ dw_status dw_logpsi(dw_model* m, const float* r, int32_t n_walkers, const float*
                     float* out_log_psi_sqr, float* out_phase, dw_stream stream) {
   if (!m || !r || !R || !Z || !out_log_psi_sqr) return fail(DW_ERR_INVALID, "null argument");
   if (n_walkers <= 0) return fail(DW_ERR_INVALID, "n_walkers must be positive");
-  dw_status st = prepare(m, false);
+  hipStream_t s = (hipStream_t)stream;
+  dw_status st = prepare(m, false, s);
   if (st) return st;
-  return pass(m, r, n_walkers, R, Z, 1, (hipStream_t)stream, out_log_psi_sqr, out_phase, nullptr, nullptr, 0.f);
+  HIP_TRY(m->kappa_w.ensure((size_t)n_walkers * 4));
+  m->kappa_n = n_walkers;
+  return pass(m, r, n_walkers, R, Z, 1, s, out_log_psi_sqr, out_phase, nullptr, nullptr, m->kappa_w.as<float>(), 0.f);
 }
 
 dw_status dw_local_energy(dw_model* m, const float* r, int32_t n_walkers, const float* R, const float* Z,
@@ -746,11 +830,13 @@ dw_status dw_local_energy(dw_model* m, const float* r, int32_t n_walkers, const 
   if (!m || !r || !R || !Z || !out_eloc) return fail(DW_ERR_INVALID, "null argument");
   if (n_walkers <= 0) return fail(DW_ERR_INVALID, "n_walkers must be positive");
   const bool recompute = m->prec != GEMM_F64 && m->d.fp64_cond_threshold > 0.f;
-  dw_status st = prepare(m, recompute);
-  if (st) return st;
   hipStream_t s = (hipStream_t)stream;
+  dw_status st = prepare(m, recompute, s);
+  if (st) return st;
   const int C = 3 * m->n + 2;
   const long chunk = m->chunk > 0 ? m->chunk : n_walkers;
+  HIP_TRY(m->kappa_w.ensure((size_t)n_walkers * 4));
+  m->kappa_n = n_walkers;
   // flags drive the recompute decision even when the caller does not ask for them
   uint32_t* flags = out_flags;
   if (recompute && !flags) {
@@ -760,7 +846,7 @@ dw_status dw_local_energy(dw_model* m, const float* r, int32_t n_walkers, const 
   for (long b0 = 0; b0 < n_walkers; b0 += chunk) {
     const long nb = std::min<long>(chunk, n_walkers - b0);
     st = pass(m, r + b0 * m->n * 3, nb, R, Z, C, s, out_log_psi_sqr ? out_log_psi_sqr + b0 : nullptr, nullptr,
-              out_eloc + b0, flags ? flags + b0 : nullptr, m->d.fp64_cond_threshold);
+              out_eloc + b0, flags ? flags + b0 : nullptr, m->kappa_w.as<float>() + b0, m->d.fp64_cond_threshold);
     if (st) return st;
   }
   if (!recompute) return DW_OK;
@@ -782,8 +868,12 @@ dw_status dw_local_energy(dw_model* m, const float* r, int32_t n_walkers, const 
   const long c2 = m->chunk > 0 ? m->chunk : cntw;
   for (long b0 = 0; b0 < cntw; b0 += c2) {
     const long nb = std::min<long>(c2, cntw - b0);
-    st = run_pass<double>(m, m->sub_r.as<float>() + b0 * m->n * 3, nb, R, Z, C, s, m->sub_lp.as<float>() + b0,
-                          nullptr, m->sub_e.as<float>() + b0, nullptr, 0.f);
+    PassBufs<double> pb;
+    st = pass_scratch(m, nb, C, &pb);
+    if (st) return st;
+    // the conditioning record keeps the FP32 estimate that flagged the walker
+    st = run_pass<double>(m, pb, m->sub_r.as<float>() + b0 * m->n * 3, nb, R, Z, C, s, m->sub_lp.as<float>() + b0,
+                          nullptr, m->sub_e.as<float>() + b0, nullptr, nullptr, 0.f);
     if (st) return st;
   }
   launch_scatter_values(m->sub_e.as<float>(), m->sub_idx.as<int32_t>(), cntw, out_eloc, s);
@@ -793,38 +883,22 @@ dw_status dw_local_energy(dw_model* m, const float* r, int32_t n_walkers, const 
   return DW_OK;
 }
 
+dw_status dw_energy_moments(const float* e_loc, int32_t n_walkers, double* out_mean_var, dw_stream stream) {
+  if (!e_loc || !out_mean_var) return fail(DW_ERR_INVALID, "null argument");
+  if (n_walkers <= 0) return fail(DW_ERR_INVALID, "n_walkers must be positive");
+  launch_energy_moments(e_loc, n_walkers, out_mean_var, (hipStream_t)stream);
+  HIP_TRY(hipGetLastError());
+  return DW_OK;
+}
+
 dw_status dw_mcmc_steps(dw_model* m, float* r, int32_t n_walkers, const float* R, const float* Z, float* log_psi_sqr,
                         int32_t* walker_age, const float* normals, const float* uniforms, int32_t n_steps,
                         dw_chain_scalars* chain, int32_t max_age, int32_t stepsize_update_interval,
                         float target_acceptance, float min_stepsize, float max_stepsize, float* out_acc,
                         uint32_t* out_flags, dw_stream stream) {
-  if (!m || !r || !R || !Z || !log_psi_sqr || !walker_age || !chain) return fail(DW_ERR_INVALID, "null argument");
-  if (n_walkers <= 0 || n_steps < 0) return fail(DW_ERR_INVALID, "bad sizes");
-  if (n_steps > 0 && (!normals || !uniforms)) return fail(DW_ERR_INVALID, "draws required");
-  dw_status st = prepare(m, false);
-  if (st) return st;
-  hipStream_t s = (hipStream_t)stream;
-  const long B = n_walkers;
-  const int cnt3 = (int)(B * m->n * 3);
-  HIP_TRY(m->rprop.ensure((size_t)cnt3 * 4));
-  HIP_TRY(m->lpprop.ensure((size_t)B * 4));
-  HIP_TRY(m->cnt.ensure((size_t)std::max(n_steps, 1) * 4));
-  // log_psi_sqr is re-evaluated on entry (mcmc.py:396)
-  st = pass(m, r, B, R, Z, 1, s, log_psi_sqr, nullptr, nullptr, nullptr, 0.f);
-  if (st) return st;
-  if (n_steps == 0) return DW_OK;
-  launch_fill_i32(m->cnt.as<int32_t>(), 0, n_steps, s);
-  for (int k = 0; k < n_steps; ++k) {
-    launch_propose(r, normals + (size_t)k * cnt3, chain, m->rprop.as<float>(), cnt3, s);
-    st = pass(m, m->rprop.as<float>(), B, R, Z, 1, s, m->lpprop.as<float>(), nullptr, nullptr, nullptr, 0.f);
-    if (st) return st;
-    launch_accept(r, m->rprop.as<float>(), log_psi_sqr, m->lpprop.as<float>(), uniforms + (size_t)k * B, walker_age,
-                  max_age, m->cnt.as<int32_t>() + k, out_flags, (int)B, m->n, s);
-    launch_chain_update(chain, m->cnt.as<int32_t>() + k, (int)B, out_acc ? out_acc + k : nullptr,
-                        stepsize_update_interval, target_acceptance, min_stepsize, max_stepsize, s);
-  }
-  HIP_TRY(hipGetLastError());
-  return DW_OK;
+  const McmcArgs a{r, n_walkers, R, Z, log_psi_sqr, walker_age, normals, uniforms, n_steps, chain, max_age,
+                   stepsize_update_interval, target_acceptance, min_stepsize, max_stepsize, out_acc, out_flags};
+  return dwmi::mcmc_run(m, a, nullptr, (hipStream_t)stream);
 }
 
 }  // extern "C"

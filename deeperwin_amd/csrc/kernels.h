@@ -91,6 +91,8 @@ void launch_combine(const double* ld, const double* sd, const double* g, const d
                     float* out_eloc, uint32_t* out_flags, float* out_kappa, const Shape& sh, hipStream_t s);
 
 void launch_fill_i32(int32_t* p, int32_t v, int count, hipStream_t s);
+// out[0] = nanmean(e[0:n]), out[1] = nanmean(|e - out[0]|^2), FP64, deterministic order.
+void launch_energy_moments(const float* e, int n, double* out, hipStream_t s);
 void launch_gather_walkers(const float* r, const int32_t* idx, int count, int n, float* out, hipStream_t s);
 void launch_scatter_values(const float* v, const int32_t* idx, int count, float* out, hipStream_t s);
 void launch_propose(const float* r, const float* normals, const void* chain, float* r_prop, int count, hipStream_t s);

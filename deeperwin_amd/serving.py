@@ -29,6 +29,7 @@ from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
+from ._native import energy_moments
 from .params import ModelShape, init_params, truncate_params
 from .physics import generate_el_ion_mapping, init_walkers_gaussian, random_molecule
 
@@ -207,12 +208,9 @@ class _Active:
                 self.steps_done += k
             if self.steps_done >= self.req.n_steps:
                 self.model.local_energy(self.r, self.R, self.Z, self.e)
-                ed = self.e.double()
-                ok = torch.isfinite(ed)
-                cnt = ok.sum()
-                mean = torch.where(ok, ed, torch.zeros_like(ed)).sum() / cnt
-                var = torch.where(ok, (ed - mean) ** 2, torch.zeros_like(ed)).sum() / cnt
-                self.moments = torch.stack([mean, var, self.acc[:max(1, self.req.n_steps)].double().mean()])
+                # nanmean semantics of get_observables (evaluation.py:143-151), FP64 on the device
+                mv = energy_moments(self.e)
+                self.moments = torch.cat([mv, self.acc[:max(1, self.req.n_steps)].double().mean().reshape(1)])
                 # result to pinned host memory on the request's own stream: reading it after the event
                 # needs no further synchronisation
                 self.host = torch.empty(3, dtype=torch.float64, pin_memory=True)

@@ -105,9 +105,16 @@ dw_status dw_mcmc_steps(dw_model* m, float* r, int32_t n_walkers, const float* R
                         int32_t stepsize_update_interval, float target_acceptance, float min_stepsize,
                         float max_stepsize, float* out_acc, uint32_t* out_flags, dw_stream stream);
 
-/* Copy the per-walker conditioning estimate of the last pass into out (device, n_walkers floats, ordered on
- * `stream`): max_d |A_d|_1 |A_d^-1|_1 (Laplacian pass; pivot ratio in the value pass) times
- * sum_d |s_d e^{l_d-m}| / |sum_d s_d e^{l_d-m}|.  For dw_local_energy with a chunk cap, the last chunk. */
+/* E_mean and E_var of a batch of local energies in FP64: out_mean_var (device, 2 doubles) receives
+ * nanmean(E_loc) and nanmean(|E_loc - E_mean|^2).  NaN walkers are dropped, +-inf kept, as jnp.nanmean
+ * does.  Replaces get_observables' energy metrics (optimization/evaluation.py:143-151; pmean over one
+ * device is the identity).  The summation order is fixed, so the result does not vary between runs. */
+dw_status dw_energy_moments(const float* e_loc, int32_t n_walkers, double* out_mean_var, dw_stream stream);
+
+/* Copy the per-walker conditioning estimate of the last dw_logpsi or dw_local_energy call into out (device,
+ * n_walkers floats, at most that call's walkers, ordered on `stream`): max_d |A_d|_1 |A_d^-1|_1 (local
+ * energy; pivot ratio in dw_logpsi) times sum_d |s_d e^{l_d-m}| / |sum_d s_d e^{l_d-m}|, for every chunk of
+ * a chunked dw_local_energy.  Walkers recomputed in FP64 keep the FP32 estimate that flagged them. */
 dw_status dw_model_conditioning(dw_model* m, float* out, int32_t n_walkers, dw_stream stream);
 
 /* Bytes of device workspace the model holds after sizing for n_walkers in the given mode

@@ -24,10 +24,13 @@ class ChainState:
     acc_rate: float = 0.0
 
 
-def adjust_stepsize(stepsize: float, acc_rate: float, target: float, lo: float, hi: float) -> float:
-    """mcmc.py:381-387: /1.05 below the target acceptance, *1.05 otherwise, then clip."""
-    s = stepsize / 1.05 if acc_rate < target else stepsize * 1.05
-    return float(np.clip(s, lo, hi))
+def adjust_stepsize(stepsize: float, acc_rate: float, target: float, lo: float, hi: float, dtype=np.float64) -> float:
+    """mcmc.py:381-387: /1.05 below the target acceptance, *1.05 otherwise, then clip.  The arithmetic is in
+    `dtype`: in the reference's float32 mode the stepsize is a float32 array and the Python constants are
+    weakly typed, so the factor is float32(1.05) and the quotient is rounded to float32."""
+    s, f = dtype(stepsize), dtype(1.05)
+    s = s / f if acc_rate < target else s * f
+    return float(np.clip(s, dtype(lo), dtype(hi)))
 
 
 def mcmc_steps(logpsi_fn: Callable[[np.ndarray], np.ndarray], state: ChainState, normals: np.ndarray,
@@ -60,7 +63,7 @@ def mcmc_steps(logpsi_fn: Callable[[np.ndarray], np.ndarray], state: ChainState,
         old_rate = acc_rate
         acc_rate = dtype(0.9 * old_rate + 0.1 * acc)
         if step_nr % update_interval == 0:
-            stepsize = dtype(adjust_stepsize(float(stepsize), float(old_rate), target, lo, hi))
+            stepsize = dtype(adjust_stepsize(float(stepsize), float(old_rate), target, lo, hi, dtype))
         accs.append(acc)
         masks.append(accept)
     out = ChainState(r=r, log_psi_sqr=lp, walker_age=age, stepsize=float(stepsize), step_nr=step_nr,

@@ -50,3 +50,17 @@ def test_unit_flops_match_published_counts():
     assert laplacian_flops(n2) == pytest.approx(0.879e9, rel=2e-3)
     assert unit_flops(lih, 10) == pytest.approx(0.126e9, rel=5e-3)
     assert unit_flops(n2, 20) == pytest.approx(1.291e9, rel=2e-3)
+
+
+def test_executed_flops_follow_the_row_split():
+    """The library multiplies the spin-mean blocks once per walker instead of once per electron: per layer
+    the executed count saves 2 (n - 1) (2 d) W1 FLOPs per value pass, and C = 3 n + 2 times that per
+    Laplacian pass (the 1-el part); everything else is counted the same in the value pass."""
+    from deeperwin_amd.batching import executed_forward_flops, executed_laplacian_flops
+
+    for shape in (ModelShape(4, 2, 2, 4), ModelShape(14, 7, 2, 16), ModelShape(26, 13, 6, 16)):
+        n, I, W1 = shape.n_el, shape.n_ion, shape.width1
+        d = [4 * I] + [W1] * (shape.n_iter - 1)
+        saved = sum(2 * (n - 1) * 2 * dl * W1 for dl in d)
+        assert forward_flops(shape) - executed_forward_flops(shape) == pytest.approx(saved, rel=1e-12)
+        assert executed_laplacian_flops(shape) < laplacian_flops(shape)
