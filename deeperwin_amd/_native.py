@@ -53,6 +53,8 @@ SIGNATURES = {
     "dw_local_energy": ([_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
     "dw_mcmc_steps": ([_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _f32, _f32, _f32, _vp,
                        _vp, _vp], ctypes.c_int),
+    "dw_mcmc_steps_noalloc": ([_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _f32, _f32, _f32,
+                               _vp, _vp, _vp], ctypes.c_int),
     "dw_energy_moments": ([_vp, _i32, _vp, _vp], ctypes.c_int),
     "dw_workspace_size": ([_vp, _i32, _i32], ctypes.c_size_t),
     "dw_model_set_chunk": ([_vp, _i32], ctypes.c_int),
@@ -227,9 +229,12 @@ class NativeModel:
                                         _ptr(out_lp), _ptr(out_flags), _stream()))
 
     def mcmc_steps(self, r, R, Z, lp, age, normals, uniforms, chain, max_age, interval, target, lo, hi,
-                   out_acc=None, out_flags=None) -> None:
+                   out_acc=None, out_flags=None, noalloc: bool = False) -> None:
+        """dw_mcmc_steps, or with noalloc=True dw_mcmc_steps_noalloc (no device allocation: every buffer of
+        the call comes from the library's fixed scratch array)."""
         n_steps = 0 if normals is None else normals.shape[0]
-        _check(self.lib.dw_mcmc_steps(self.handle, _ptr(r), r.shape[0], _ptr(R), _ptr(Z), _ptr(lp), _ptr(age),
+        fn = self.lib.dw_mcmc_steps_noalloc if noalloc else self.lib.dw_mcmc_steps
+        _check(fn(self.handle, _ptr(r), r.shape[0], _ptr(R), _ptr(Z), _ptr(lp), _ptr(age),
                                       _ptr(normals), _ptr(uniforms), n_steps, _ptr(chain), int(max_age),
                                       int(interval), float(target), float(lo), float(hi), _ptr(out_acc),
                                       _ptr(out_flags), _stream()))

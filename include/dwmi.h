@@ -105,6 +105,22 @@ dw_status dw_mcmc_steps(dw_model* m, float* r, int32_t n_walkers, const float* R
                         int32_t stepsize_update_interval, float target_acceptance, float min_stepsize,
                         float max_stepsize, float* out_acc, uint32_t* out_flags, dw_stream stream);
 
+/* dw_mcmc_steps without any device allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, kernels and results as dw_mcmc_steps, but every device buffer of the call
+ * (activations of the value pass, proposals, acceptance counters) is carved from one static device array
+ * of DW_NOALLOC_SCRATCH_BYTES per GPU, held from library load to process end; the call makes no hipMalloc
+ * or hipFree.  Fails with DW_ERR_INVALID when n_walkers do not fit that array (the message states how
+ * many fit) and with DW_ERR_STATE when the weights are not on the device yet (call dw_logpsi, dw_mcmc_steps
+ * or dw_local_energy once after the last dw_model_set_param).  Every call on one GPU shares the array:
+ * calls that overlap in time on one GPU race.  It does not update the dw_model_conditioning record. */
+#define DW_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+dw_status dw_mcmc_steps_noalloc(dw_model* m, float* r, int32_t n_walkers, const float* R, const float* Z,
+                                float* log_psi_sqr, int32_t* walker_age, const float* normals, const float* uniforms,
+                                int32_t n_steps, dw_chain_scalars* chain, int32_t max_age,
+                                int32_t stepsize_update_interval, float target_acceptance, float min_stepsize,
+                                float max_stepsize, float* out_acc, uint32_t* out_flags, dw_stream stream);
+
 /* E_mean and E_var of a batch of local energies in FP64: out_mean_var (device, 2 doubles) receives
  * nanmean(E_loc) and nanmean(|E_loc - E_mean|^2).  NaN walkers are dropped, +-inf kept, as jnp.nanmean
  * does.  Replaces get_observables' energy metrics (optimization/evaluation.py:143-151; pmean over one
