@@ -89,3 +89,24 @@ def test_workspace_size_formula():
     assert 0 < v < lap
     m.set_chunk(512)
     assert m.workspace_bytes(2048, 1) < lap
+
+
+def test_noalloc_mcmc_rejects_before_touching_the_device():
+    """dw_mcmc_steps_noalloc validates sizes and weight state on the host first: too many walkers for its
+    fixed scratch array -> DW_ERR_INVALID naming the limit; weights not uploaded yet -> DW_ERR_STATE.
+    The pointers below are never dereferenced."""
+    shape = ModelShape(14, 7, 2, 16)
+    m = _native.NativeModel(shape)
+    m.set_params(init_params(shape, 1))
+    fake = ctypes.c_void_p(256)
+    lib = m.lib
+
+    def call(B):
+        return lib.dw_mcmc_steps_noalloc(m.handle, fake, B, fake, fake, fake, fake, None, None, 0, fake, 100, 100,
+                                         0.5, 0.01, 1.0, None, None, None)
+
+    assert call(1 << 20) == 1
+    msg = lib.dw_last_error().decode()
+    assert "walkers" in msg and "fit" in msg
+    assert call(64) == 3
+    assert "not on the device" in lib.dw_last_error().decode()
