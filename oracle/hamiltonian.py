@@ -63,8 +63,36 @@ def kinetic_energy(params: Dict, n_up: int, n_dn: int, r: torch.Tensor, R: torch
     return torch.stack([one_walker(x[b]) for b in range(x.shape[0])])
 
 
+def kinetic_energy_vmapped(params: Dict, n_up: int, n_dn: int, r: torch.Tensor, R: torch.Tensor,
+                           dtype=torch.float64) -> torch.Tensor:
+    """kinetic_energy with the walkers of r [B, n, 3] batched by vmap, as folx.batched_vmap batches a chunk
+    in the reference (hamiltonian.py:280).  Only for models with two or more determinants: with a single
+    determinant per walker the vmapped forward-over-reverse of slogdet is wrong in this PyTorch build (see
+    kinetic_energy), so those fall back to the walker loop."""
+    from deeperwin_amd.params import infer_shape
+
+    n = n_up + n_dn
+    if infer_shape(params, n, n_up).n_det < 2:
+        return kinetic_energy(params, n_up, n_dn, r, R, dtype=dtype)
+    R = R.to(dtype)
+
+    def f(x):
+        return wf.log_psi_sqr(params, n_up, n_dn, x.reshape(1, n, 3), R, dtype=dtype)[1][0]
+
+    grad_f = torch.func.grad(f)
+    eye = torch.eye(3 * n, dtype=dtype)
+
+    def one_walker(x):
+        g = grad_f(x)
+        hess_cols = torch.func.vmap(lambda v: torch.func.jvp(grad_f, (x,), (v,))[1])(eye)
+        lap = torch.diagonal(hess_cols).sum()
+        return -0.5 * (0.25 * (g ** 2).sum() + 0.5 * lap)
+
+    return torch.func.vmap(one_walker)(r.reshape(r.shape[0], 3 * n).to(dtype))
+
+
 def local_energy(params: Dict, n_up: int, n_dn: int, r, R, Z, max_batch_size: int = 64,
-                 dtype=torch.float64) -> torch.Tensor:
+                 dtype=torch.float64, vmapped: bool = False) -> torch.Tensor:
     r = torch.as_tensor(np.asarray(r)) if not isinstance(r, torch.Tensor) else r
     R = torch.as_tensor(np.asarray(R)) if not isinstance(R, torch.Tensor) else R
     Z = torch.as_tensor(np.asarray(Z)) if not isinstance(Z, torch.Tensor) else Z
@@ -72,7 +100,8 @@ def local_energy(params: Dict, n_up: int, n_dn: int, r, R, Z, max_batch_size: in
     out = []
     for s in range(0, r.shape[0], max_batch_size):
         rc = r[s:s + max_batch_size]
-        out.append(kinetic_energy(params, n_up, n_dn, rc, R, dtype=dtype) + potential_energy(rc, R.to(dtype), Z))
+        kin = kinetic_energy_vmapped if vmapped else kinetic_energy
+        out.append(kin(params, n_up, n_dn, rc, R, dtype=dtype) + potential_energy(rc, R.to(dtype), Z))
     return torch.cat(out)
 
 

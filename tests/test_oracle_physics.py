@@ -190,3 +190,16 @@ def test_pinning_table_names_existing_tests():
             tree = ast.parse(open(os.path.join(here, fname)).read())
             names = {n.name for n in ast.walk(tree) if isinstance(n, ast.FunctionDef)}
             assert test in names, (fn, pin)
+
+
+def test_vmapped_kinetic_energy_equals_walker_loop():
+    """The chunk-batched kinetic energy (vmap over walkers, the reference's batched_vmap) equals the walker
+    loop; one determinant falls back to the loop."""
+    from workloads import system, walkers
+
+    for name, D in (("lih", 3), ("he", 1)):
+        shape, params, R, Z, mapping = system(name, n_det=D, seed=2)
+        r = torch.tensor(walkers(R, mapping, 5, seed=4).astype(np.float64))
+        a = oh.kinetic_energy(params, shape.n_up, shape.n_dn, r, torch.tensor(R))
+        b = oh.kinetic_energy_vmapped(params, shape.n_up, shape.n_dn, r, torch.tensor(R))
+        np.testing.assert_allclose(b.numpy(), a.numpy(), rtol=1e-11, atol=1e-11)
