@@ -545,7 +545,9 @@ __global__ __launch_bounds__(256) void k_gemm_lap(const float* __restrict__ X, i
       if (k0 + BK < K) gload(k0 + BK);
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
-        const int kr = kk + (lane >> 4);
+        // MFMA step kk / 4 of lane group g = lane / 16 takes k = kk / 4 + 4 g (A and B alike): the four rows
+        // one read touches lie 4 k apart, so with the pitch of 4 (mod 64) floats they start 16 banks apart
+        const int kr = (kk >> 2) + 4 * (lane >> 4);
         float bq[NQ];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) bq[q] = Bs[kr][wv * WN + q * 16 + (lane & 15)];
