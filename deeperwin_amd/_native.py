@@ -56,6 +56,11 @@ SIGNATURES = {
     "dw_mcmc_steps_noalloc": ([_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _f32, _f32, _f32,
                                _vp, _vp, _vp], ctypes.c_int),
     "dw_energy_moments": ([_vp, _i32, _vp, _vp], ctypes.c_int),
+    "dw_param_grad": ([_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "dw_param_count": ([_vp], ctypes.c_int64),
+    "dw_clip_energies": ([_vp, _i32, _i32, _i32, _i32, _f32, _i32, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "dw_param_layout": ([_vp, _i32, ctypes.c_char_p, _i32, ctypes.POINTER(ctypes.c_int64),
+                         ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     "dw_workspace_size": ([_vp, _i32, _i32], ctypes.c_size_t),
     "dw_model_set_chunk": ([_vp, _i32], ctypes.c_int),
     "dw_model_set_workspace": ([_vp, _vp, ctypes.c_size_t], ctypes.c_int),
@@ -216,6 +221,46 @@ class NativeModel:
         _check(self.lib.dw_model_conditioning(self.handle, _ptr(out), int(n_walkers), _stream()))
         return out
 
+    def param_layout(self):
+        """[(module suffix, leaf, offset, shape)] of the packed parameter gradient, in order."""
+        out = []
+        key = ctypes.create_string_buffer(128)
+        off = ctypes.c_int64()
+        shp = (ctypes.c_int64 * 2)()
+        idx = 0
+        while self.lib.dw_param_layout(self.handle, idx, key, 128, ctypes.byref(off), shp) == 0:
+            mod, leaf = key.value.decode().split(":")
+            shape = (int(shp[0]),) if leaf == "b" else (int(shp[0]), int(shp[1]))
+            out.append((mod, leaf, int(off.value), shape))
+            idx += 1
+        return out
+
+    def param_grad(self, r, R, Z, walker_weights, out_lp=None):
+        """Packed G = sum_b c_b d log|psi|^2(r_b) / d theta as a CUDA float32 tensor (dw_param_grad)."""
+        import torch
+
+        g = torch.empty(int(self.lib.dw_param_count(self.handle)), device="cuda", dtype=torch.float32)
+        c = walker_weights.to(torch.float32).contiguous()
+        _check(self.lib.dw_param_grad(self.handle, _ptr(r), r.shape[0], _ptr(R), _ptr(Z), _ptr(c), _ptr(g),
+                                      _ptr(out_lp), _stream()))
+        return g
+
+    def unpack_grad(self, g, params: Dict) -> Dict:
+        """The packed gradient as a dict shaped like `params` (module paths matched by suffix)."""
+        out = {}
+        layout = {(mod, leaf): (off, shape) for mod, leaf, off, shape in self.param_layout()}
+        for path, leaves in params.items():
+            norm = "/".join(seg for seg in path.split("/") if seg != "~")
+            out[path] = {}
+            for leaf in leaves:
+                hit = [k for k in layout if k[1] == leaf and (norm == k[0] or norm.endswith("/" + k[0]))]
+                if len(hit) != 1:
+                    raise KeyError(f"{path}/{leaf}: {len(hit)} gradient leaves match")
+                off, shape = layout[hit[0]]
+                cnt = int(np.prod(shape))
+                out[path][leaf] = g[off:off + cnt].reshape(shape)
+        return out
+
     def workspace_bytes(self, n_walkers: int, mode: int) -> int:
         return int(self.lib.dw_workspace_size(self.handle, int(n_walkers), int(mode)))
 
@@ -250,3 +295,22 @@ def energy_moments(e_loc):
     e = e_loc.reshape(-1).to(torch.float32).contiguous()
     _check(load_library().dw_energy_moments(_ptr(e), int(e.numel()), _ptr(out), _stream()))
     return out
+
+
+def clip_energies(e_loc, state=None, name: str = "tanh", center: str = "mean", width_metric: str = "std",
+                  clip_by: float = 5.0, from_previous_step: bool = True):
+    """dw_clip_energies on a CUDA tensor of local energies: (per-walker gradient weights, clipped energies,
+    stats [loss, E_mean, E_var, E_var_clipped, next center, next width, center used, width used]) as CUDA
+    tensors.  `state` is a CUDA float64 tensor (center, width) or None."""
+    import torch
+
+    e = e_loc.reshape(-1).to(torch.float32).contiguous()
+    w = torch.empty_like(e)
+    ec = torch.empty_like(e)
+    stats = torch.empty(8, device=e.device, dtype=torch.float64)
+    st = None if state is None else state.to(device=e.device, dtype=torch.float64).contiguous()
+    _check(load_library().dw_clip_energies(_ptr(e), int(e.numel()), 1 if name == "tanh" else 0,
+                                           1 if center == "median" else 0, 1 if width_metric == "mae" else 0,
+                                           float(clip_by), 1 if from_previous_step else 0, _ptr(st), _ptr(w), _ptr(ec),
+                                           _ptr(stats), _stream()))
+    return w, ec, stats

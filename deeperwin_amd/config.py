@@ -121,6 +121,25 @@ class MCMCConfig:
 
 
 @dataclass
+class ClippingConfig:
+    """Local-energy clipping of the loss (configuration.py:1052-1058)."""
+    name: str = "tanh"
+    width_metric: str = "std"
+    center: str = "mean"
+    from_previous_step: bool = True
+    clip_by: float = 5.0
+    clip_imag_around_0: bool = False
+
+    def validate(self) -> None:
+        if self.name not in ("hard", "tanh"):
+            raise ValueError(f"Unsupported config-value for optimization.clipping.name: {self.name}")
+        if self.width_metric not in ("std", "mae"):
+            raise NotImplementedError(f"Unknown clipping metric: {self.width_metric}")
+        if self.center not in ("mean", "median"):
+            raise NotImplementedError(f"Unknown clipping center: {self.center}")
+
+
+@dataclass
 class PhysicalConfig:
     """Molecule description (configuration.py:1537-1745): R in bohr, integer charges Z, electron counts."""
     R: List[List[float]]

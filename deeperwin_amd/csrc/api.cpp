@@ -15,6 +15,7 @@
 
 #include "api_internal.h"
 #include "dwmi.h"
+#include "grad.h"
 #include "kernels.h"
 
 using namespace dwmi;
@@ -109,11 +110,14 @@ int rows_per_electron(int C) { return C > 1 ? pad_to(C, 16) : 1; }
 struct Weights {
   bool ready = false;
   std::vector<DevBuf> Wmain, Wmu, bias, Wmain_hi, Wmain_lo, Wmu_hi, Wmu_lo;
-  DevBuf W2e, b2e, Wo[2], Wo_hi[2], Wo_lo[2], kappa, wenv;
+  std::vector<DevBuf> WmainT, WmuT;  // transposed copies for the gradient pass ([pad128(K)][W1])
+  DevBuf W2e, b2e, Wo[2], Wo_hi[2], Wo_lo[2], WoT[2], kappa, wenv;
   void release() {
-    for (auto* v : {&Wmain, &Wmu, &bias, &Wmain_hi, &Wmain_lo, &Wmu_hi, &Wmu_lo})
+    for (auto* v : {&Wmain, &Wmu, &bias, &Wmain_hi, &Wmain_lo, &Wmu_hi, &Wmu_lo, &WmainT, &WmuT})
       for (auto& b : *v) b.release();
-    for (DevBuf* b : {&W2e, &b2e, &Wo[0], &Wo[1], &Wo_hi[0], &Wo_hi[1], &Wo_lo[0], &Wo_lo[1], &kappa, &wenv}) b->release();
+    for (DevBuf* b : {&W2e, &b2e, &Wo[0], &Wo[1], &Wo_hi[0], &Wo_hi[1], &Wo_lo[0], &Wo_lo[1], &WoT[0], &WoT[1], &kappa,
+                      &wenv})
+      b->release();
     ready = false;
   }
 };
@@ -145,7 +149,7 @@ struct Carver {
 
 struct dw_model {
   dw_model_desc d{};
-  int n = 0, u = 0, dn = 0, I = 0, D = 0, W1 = 0, W2 = 0, L = 0, NO = 0, NOp = 0;
+  int n = 0, u = 0, dn = 0, I = 0, D = 0, W1 = 0, W2 = 0, L = 0, NO = 0, NOp = 0, NOk = 0;
   std::vector<int> Kh, Ke, Kx, Km;
   std::map<std::string, Leaf> leaves;  // key "<module>:<leaf>"
   std::vector<std::pair<std::string, std::vector<int64_t>>> expected;
@@ -157,6 +161,7 @@ struct dw_model {
   GemmPrec prec = GEMM_F32;
   Weights wf, wd;    // float and double layouts
   DevBuf arena_f, arena_d;  // pass scratch (PassBufs) of the float and double activation types
+  DevBuf grad_arena;        // scratch of the parameter-gradient pass
   DevBuf kappa_w;           // per-walker conditioning of the last pass (dw_model_conditioning)
   long kappa_n = 0;         // walkers covered by kappa_w
   DevBuf rprop, lpprop, cnt;
@@ -178,7 +183,7 @@ struct dw_model {
     for (auto e : event_pool) (void)hipEventDestroy(e);
     wf.release();
     wd.release();
-    for (DevBuf* b : {&arena_f, &arena_d, &kappa_w, &rprop, &lpprop, &cnt, &sub_r, &sub_e, &sub_lp,
+    for (DevBuf* b : {&arena_f, &arena_d, &grad_arena, &kappa_w, &rprop, &lpprop, &cnt, &sub_r, &sub_e, &sub_lp,
                       &sub_idx, &own_flags})
       b->release();
   }
@@ -237,7 +242,8 @@ dw_status prepare_weights(dw_model* m, Weights& W, bool dbl) {
     if (!m->leaves[e.first].set) return fail(DW_ERR_STATE, "parameter not set: " + e.first);
   const int W1 = m->W1, W2 = m->W2;
   const bool split = !dbl && m->prec != GEMM_F32;
-  for (auto* v : {&W.Wmain, &W.Wmu, &W.bias, &W.Wmain_hi, &W.Wmain_lo, &W.Wmu_hi, &W.Wmu_lo}) v->resize(m->L);
+  for (auto* v : {&W.Wmain, &W.Wmu, &W.bias, &W.Wmain_hi, &W.Wmain_lo, &W.Wmu_hi, &W.Wmu_lo, &W.WmainT, &W.WmuT})
+    v->resize(m->L);
   auto up = [&](DevBuf& b, const std::vector<float>& v) { return dbl ? upload_as<double>(b, v) : upload(b, v); };
   for (int l = 0; l < m->L; ++l) {
     char buf[64];
@@ -252,9 +258,17 @@ dw_status prepare_weights(dw_model* m, Weights& W, bool dbl) {
       for (int k = 0; k < 2 * Ke; ++k) wm[(size_t)o * Kx + Kh + k] = w.data[(size_t)(3 * Kh + k) * W1 + o];
       for (int k = 0; k < 2 * Kh; ++k) wu[(size_t)o * Km + k] = w.data[(size_t)(Kh + k) * W1 + o];
     }
+    // transposed copies [pad128(K)][W1] for the gradient pass (dX = dz W^T as C = A Bt^T)
+    std::vector<float> wmT((size_t)pad_to(Kx, 128) * W1, 0.f), wuT((size_t)pad_to(Km, 128) * W1, 0.f);
+    for (int o = 0; o < W1; ++o) {
+      for (int k = 0; k < Kx; ++k) wmT[(size_t)k * W1 + o] = wm[(size_t)o * Kx + k];
+      for (int k = 0; k < Km; ++k) wuT[(size_t)k * W1 + o] = wu[(size_t)o * Km + k];
+    }
     dw_status st;
     if ((st = up(W.Wmain[l], wm))) return st;
     if ((st = up(W.Wmu[l], wu))) return st;
+    if ((st = up(W.WmainT[l], wmT))) return st;
+    if ((st = up(W.WmuT[l], wuT))) return st;
     if ((st = up(W.bias[l], b.data))) return st;
     if (split) {
       if ((st = upload_split(W.Wmain_hi[l], W.Wmain_lo[l], wm))) return st;
@@ -287,8 +301,12 @@ dw_status prepare_weights(dw_model* m, Weights& W, bool dbl) {
     std::vector<float> wo((size_t)m->NOp * W1, 0.f);
     for (int c = 0; c < m->NO; ++c)
       for (int k = 0; k < W1; ++k) wo[(size_t)c * W1 + k] = w.data[(size_t)k * m->NO + c];
+    std::vector<float> woT((size_t)pad_to(W1, 128) * m->NOk, 0.f);
+    for (int c = 0; c < m->NO; ++c)
+      for (int k = 0; k < W1; ++k) woT[(size_t)k * m->NOk + c] = w.data[(size_t)k * m->NO + c];
     dw_status st;
     if ((st = up(W.Wo[s], wo))) return st;
+    if ((st = up(W.WoT[s], woT))) return st;
     if (split && (st = upload_split(W.Wo_hi[s], W.Wo_lo[s], wo))) return st;
     const Leaf& a = m->leaves[leaf_key("envelope_orbitals", ak[s])];
     const Leaf& we = m->leaves[leaf_key("envelope_orbitals", wk[s])];
@@ -530,6 +548,142 @@ dw_status run_pass(dw_model* m, const PassBufs<T>& ws, const float* r, long B, c
   return DW_OK;
 }
 
+// Offset (floats) of a parameter leaf in the packed gradient: the leaves in the order of m->expected.
+long leaf_offset(const dw_model* m, const std::string& key) {
+  long off = 0;
+  for (auto& e : m->expected) {
+    if (e.first == key) return off;
+    long cnt = 1;
+    for (auto v : e.second) cnt *= v;
+    off += cnt;
+  }
+  return -1;
+}
+
+// Scratch of the gradient pass (after the value pass's own scratch).
+template <typename T>
+struct GradBufs {
+  double* gl = nullptr;
+  T *dbf = nullptr, *denv = nullptr, *dH = nullptr, *dz = nullptr, *DZS = nullptr, *dMU = nullptr, *MU = nullptr;
+  T* dX[4] = {nullptr, nullptr, nullptr, nullptr};
+  T* X2[3] = {nullptr, nullptr, nullptr};
+  T* DP[3] = {nullptr, nullptr, nullptr};
+  T *part = nullptr, *cpart = nullptr, *epart = nullptr;
+};
+
+constexpr int kEnvWalkersPerChunk = 32;
+
+template <typename T>
+size_t layout_grad(const dw_model* m, long B, char* base, GradBufs<T>* out) {
+  const long rows = B * m->n, pairs = B * m->n * m->n;
+  int kmmax = 0;
+  for (int l = 0; l < m->L; ++l) kmmax = std::max(kmmax, m->Km[l]);
+  Carver cv{base};
+  GradBufs<T> g;
+  g.gl = cv.take<double>((size_t)B * m->D);
+  g.dbf = cv.take<T>((size_t)rows * m->NOk);
+  g.denv = cv.take<T>((size_t)rows * m->NOk);
+  g.dH = cv.take<T>((size_t)rows * m->W1);
+  g.dz = cv.take<T>((size_t)rows * m->W1);
+  g.DZS = cv.take<T>((size_t)B * m->W1);
+  g.dMU = cv.take<T>((size_t)B * kmmax);
+  g.MU = cv.take<T>((size_t)B * kmmax);
+  for (int l = 1; l < m->L; ++l) g.dX[l] = cv.take<T>((size_t)rows * m->Kx[l]);
+  for (int l = 0; l < m->L - 1; ++l) {
+    g.X2[l] = cv.take<T>((size_t)pairs * (l == 0 ? 4 : 32));
+    g.DP[l] = cv.take<T>((size_t)pairs * 32);
+  }
+  size_t pmax = 0;
+  for (int l = 0; l < m->L; ++l) {
+    pmax = std::max(pmax, wgrad_part_elems(m->Kh[l], m->W1, rows));
+    pmax = std::max(pmax, wgrad_part_elems(2 * m->Ke[l], m->W1, rows));
+    pmax = std::max(pmax, wgrad_part_elems(2 * m->Kh[l], m->W1, B));
+  }
+  pmax = std::max(pmax, wgrad_part_elems(32, 32, pairs));
+  pmax = std::max(pmax, wgrad_part_elems(m->W1, m->NO, B * std::max(m->u, m->dn)));
+  g.part = cv.take<T>(pmax);
+  g.cpart = cv.take<T>((size_t)((std::max(pairs, B) + 511) / 512) * std::max(32, m->W1));
+  g.epart = cv.take<T>((size_t)((B + kEnvWalkersPerChunk - 1) / kEnvWalkersPerChunk) * 4 * m->I * m->NO);
+  if (out) *out = g;
+  return cv.off;
+}
+
+// Parameter gradient G = sum_b c_b d log_psi_sqr(r_b) / d theta into the packed float buffer `grad`.
+template <typename T>
+dw_status run_grad(dw_model* m, const float* r, long B, const float* R, const float* Z, const float* c, float* grad,
+                   float* out_lp, hipStream_t s) {
+  constexpr bool DBL = sizeof(T) == 8;
+  Weights& W = DBL ? m->wd : m->wf;
+  PassBufs<T> pb;
+  dw_status st = pass_scratch(m, B, 1, &pb);
+  if (st) return st;
+  st = run_pass<T>(m, pb, r, B, R, Z, 1, s, out_lp, nullptr, nullptr, nullptr, nullptr, 0.f);
+  if (st) return st;
+  HIP_TRY(m->grad_arena.ensure(layout_grad<T>(m, B, nullptr, nullptr)));
+  GradBufs<T> g;
+  layout_grad<T>(m, B, m->grad_arena.as<char>(), &g);
+  const Shape sh{(int)B, m->n, m->u, m->I, m->D, 1, 1, m->W1, m->W2, m->L};
+  const long rows = B * m->n, pairs = rows * m->n;
+  const long up_rows = B * m->u, dn_rows = B * m->dn;
+  auto mm = [&](const T* A, int lda, const T* Bt, int ldb, T* C, int ldc, long M, int N, int K) {
+    if constexpr (DBL) {
+      GemmEpiD e;
+      gemm_f64(A, lda, Bt, ldb, C, ldc, (int)M, N, K, e, s);
+    } else {
+      GemmEpi e;
+      gemm(GEMM_F32, A, lda, Bt, nullptr, nullptr, ldb, C, ldc, (int)M, N, K, e, s);
+    }
+  };
+  auto off = [&](const char* mod, const char* leaf) { return grad + leaf_offset(m, leaf_key(mod, leaf)); };
+  // determinants -> orbitals and envelope
+  launch_grad_seed(pb.ld, pb.sd, c, g.gl, sh, s);
+  const OrbSrc<T> src{pb.O, m->NO, r, R, W.kappa.as<T>(), W.wenv.as<T>()};
+  launch_det_bwd<T>(src, g.gl, g.dbf, g.denv, m->NOk, sh, s);
+  {
+    const int chunks = launch_env_grad<T>(g.denv, m->NOk, r, R, W.kappa.as<T>(), W.wenv.as<T>(), g.epart,
+                                          kEnvWalkersPerChunk, sh, s);
+    launch_sum_parts<T>(g.epart, chunks, 4L * m->I * m->NO, off("envelope_orbitals", "alpha_up"), s);
+  }
+  const char* bfk[2] = {"bf_up/linear_0", "bf_dn/linear_0"};
+  for (int sp = 0; sp < 2; ++sp) {
+    const long r0 = sp ? up_rows : 0, nr = sp ? dn_rows : up_rows;
+    wgrad<T>(pb.H + r0 * m->W1, m->W1, g.dbf + r0 * m->NOk, m->NOk, m->W1, m->NO, nr, g.part, off(bfk[sp], "w"), m->NO,
+             s);
+    mm(g.dbf + r0 * m->NOk, m->NOk, W.WoT[sp].as<T>(), m->NOk, g.dH + r0 * m->W1, m->W1, nr, m->W1, m->NOk);
+  }
+  // 1-el layers, last to first
+  launch_dz_last<T>(g.dH, pb.H, g.dz, sh, s);
+  for (int l = m->L - 1; l >= 0; --l) {
+    char key[64];
+    std::snprintf(key, sizeof key, "h_el_%d/linear_0", l);
+    const int Kh = m->Kh[l], Ke = m->Ke[l], Kx = m->Kx[l], Km = m->Km[l];
+    float* gw = off(key, "w");
+    launch_walker_sum<T>(g.dz, m->W1, m->W1, g.DZS, sh, s);
+    launch_spin_mean<T>(pb.X[l], Kx, Kh, g.MU, Km, sh, s);
+    wgrad<T>(pb.X[l], Kx, g.dz, m->W1, Kh, m->W1, rows, g.part, gw, m->W1, s);
+    wgrad<T>(g.MU, Km, g.DZS, m->W1, 2 * Kh, m->W1, B, g.part, gw + (long)Kh * m->W1, m->W1, s);
+    wgrad<T>(pb.X[l] + Kh, Kx, g.dz, m->W1, 2 * Ke, m->W1, rows, g.part, gw + 3L * Kh * m->W1, m->W1, s);
+    launch_colsum<T>(g.DZS, m->W1, B, m->W1, g.cpart, off(key, "b"), s);
+    if (l == 0) break;
+    mm(g.dz, m->W1, W.WmainT[l].as<T>(), m->W1, g.dX[l], Kx, rows, Kx, m->W1);
+    mm(g.DZS, m->W1, W.WmuT[l].as<T>(), m->W1, g.dMU, Km, B, Km, m->W1);
+    launch_dz<T>(g.dX[l], Kx, g.dMU, Km, pb.X[l], Kx, Kh, g.dz, sh, s);
+  }
+  // 2-el stream
+  if (m->L > 1) {
+    launch_twoel_bwd<T>(r, g.dX, m->Kx.data(), m->Kh.data(), W.W2e.as<T>(), W.b2e.as<T>(), g.X2, g.DP, sh, s);
+    for (int l = 0; l < m->L - 1; ++l) {
+      char key[64];
+      std::snprintf(key, sizeof key, "h_el_el_%d/linear_0", l);
+      const int kin = l == 0 ? 4 : 32;
+      wgrad<T>(g.X2[l], kin, g.DP[l], 32, kin, 32, pairs, g.part, off(key, "w"), 32, s);
+      launch_colsum<T>(g.DP[l], 32, pairs, 32, g.cpart, off(key, "b"), s);
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  return DW_OK;
+}
+
 // One pass in the model's precision, its scratch from the model (arena or caller-owned workspace).
 dw_status pass(dw_model* m, const float* r, long B, const float* R, const float* Z, int C, hipStream_t s,
                float* out_lp, float* out_phase, float* out_eloc, uint32_t* out_flags, float* out_kappa,
@@ -684,6 +838,7 @@ dw_status dw_model_create(const dw_model_desc* desc, dw_model** out) {
   m->L = d.n_iter;
   m->NO = m->D * m->n;
   m->NOp = pad_to(m->NO, 128);
+  m->NOk = pad_to(m->NO, 32);
   m->prec = (GemmPrec)d.precision;
   for (int l = 0; l < m->L; ++l) {
     const int Kh = (l == 0) ? 4 * m->I : m->W1;
@@ -881,6 +1036,57 @@ dw_status dw_local_energy(dw_model* m, const float* r, int32_t n_walkers, const 
   // the host index list is copied before this call returns
   HIP_TRY(hipStreamSynchronize(s));
   return DW_OK;
+}
+
+dw_status dw_clip_energies(const float* e_loc, int32_t n_walkers, int32_t clip_tanh, int32_t center_median,
+                           int32_t width_mae, float clip_by, int32_t from_previous_step, const double* state_in,
+                           float* out_weights, float* out_e_clipped, double* out_stats, dw_stream stream) {
+  if (!e_loc || !out_weights || !out_stats) return fail(DW_ERR_INVALID, "null argument");
+  if (n_walkers <= 0) return fail(DW_ERR_INVALID, "n_walkers must be positive");
+  if (center_median && n_walkers > clip_max_walkers())
+    return fail(DW_ERR_INVALID, "a median clipping center supports at most " + std::to_string(clip_max_walkers()) +
+                                    " walkers per device");
+  launch_clip_energies(e_loc, n_walkers, clip_tanh ? 1 : 0, center_median ? 1 : 0, width_mae ? 1 : 0, (double)clip_by,
+                       from_previous_step ? 1 : 0, state_in, out_weights, out_e_clipped, out_stats,
+                       (hipStream_t)stream);
+  HIP_TRY(hipGetLastError());
+  return DW_OK;
+}
+
+int64_t dw_param_count(const dw_model* m) {
+  if (!m) return 0;
+  int64_t tot = 0;
+  for (auto& e : m->expected) {
+    int64_t cnt = 1;
+    for (auto v : e.second) cnt *= v;
+    tot += cnt;
+  }
+  return tot;
+}
+
+dw_status dw_param_layout(const dw_model* m, int32_t index, char* key, int32_t key_len, int64_t* offset,
+                          int64_t* shape) {
+  if (!m || !key || key_len <= 0 || !offset || !shape) return fail(DW_ERR_INVALID, "null argument");
+  if (index < 0 || index >= (int32_t)m->expected.size()) return fail(DW_ERR_INVALID, "leaf index out of range");
+  const auto& e = m->expected[index];
+  std::snprintf(key, (size_t)key_len, "%s", e.first.c_str());
+  *offset = leaf_offset(m, e.first);
+  shape[0] = e.second.size() > 0 ? e.second[0] : 1;
+  shape[1] = e.second.size() > 1 ? e.second[1] : 1;
+  return DW_OK;
+}
+
+dw_status dw_param_grad(dw_model* m, const float* r, int32_t n_walkers, const float* R, const float* Z,
+                        const float* walker_weights, float* out_grad, float* out_log_psi_sqr, dw_stream stream) {
+  if (!m || !r || !R || !Z || !walker_weights || !out_grad) return fail(DW_ERR_INVALID, "null argument");
+  if (n_walkers <= 0) return fail(DW_ERR_INVALID, "n_walkers must be positive");
+  if (m->n > 52) return fail(DW_ERR_INVALID, "dw_param_grad supports up to 52 electrons (the FP64 inverse in LDS)");
+  hipStream_t s = (hipStream_t)stream;
+  dw_status st = prepare(m, false, s);
+  if (st) return st;
+  if (m->prec == GEMM_F64)
+    return run_grad<double>(m, r, n_walkers, R, Z, walker_weights, out_grad, out_log_psi_sqr, s);
+  return run_grad<float>(m, r, n_walkers, R, Z, walker_weights, out_grad, out_log_psi_sqr, s);
 }
 
 dw_status dw_energy_moments(const float* e_loc, int32_t n_walkers, double* out_mean_var, dw_stream stream) {

@@ -93,6 +93,12 @@ void launch_combine(const double* ld, const double* sd, const double* g, const d
 void launch_fill_i32(int32_t* p, int32_t v, int count, hipStream_t s);
 // out[0] = nanmean(e[0:n]), out[1] = nanmean(|e - out[0]|^2), FP64, deterministic order.
 void launch_energy_moments(const float* e, int n, double* out, hipStream_t s);
+// Clipped-energy statistics and per-walker gradient weights (optimization/loss_function.py:12-110); one
+// workgroup.  A median center needs n <= clip_max_walkers().
+int clip_max_walkers();
+void launch_clip_energies(const float* e, int n, int tanh_clip, int median, int mae, double clip_by,
+                          int from_previous, const double* state_in, float* weights, float* eclip, double* stats,
+                          hipStream_t s);
 void launch_gather_walkers(const float* r, const int32_t* idx, int count, int n, float* out, hipStream_t s);
 void launch_scatter_values(const float* v, const int32_t* idx, int count, float* out, hipStream_t s);
 void launch_propose(const float* r, const float* normals, const void* chain, float* r_prop, int count, hipStream_t s);

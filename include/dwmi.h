@@ -121,6 +121,38 @@ dw_status dw_mcmc_steps_noalloc(dw_model* m, float* r, int32_t n_walkers, const 
                                 int32_t stepsize_update_interval, float target_acceptance, float min_stepsize,
                                 float max_stepsize, float* out_acc, uint32_t* out_flags, dw_stream stream);
 
+/* ---- Parameter gradient (the optimisation step's tangent) ----
+ * G = sum_b c_b d log_psi_sqr(r_b) / d theta for every parameter leaf, with the caller's per-walker
+ * weights c (device, n_walkers floats).  With c_b = (E_clip_b - mean E_clip) / B this is the gradient that
+ * the reference's total-energy JVP produces (optimization/loss_function.py:115-156: jnp.dot of the
+ * log_psi_sqr tangents with the clipped deviations, divided by the batch size).  The result is written as
+ * float32 to out_grad (device, dw_param_count floats): the leaves one after another in the order of
+ * dw_param_layout, each row-major in the reference's shape (w: [in, out], b: [out], envelope [n_ion, D n]).
+ * out_log_psi_sqr (may be NULL) receives log_psi_sqr of the walkers.  Reverse mode through the value pass:
+ * determinants inverted in FP64, every reduction in a fixed order.  Uses its own device workspace (not the
+ * caller-owned one of dw_model_set_workspace).  NaN walker weights give NaN gradients, as in the reference.
+ * Up to 52 electrons. */
+dw_status dw_param_grad(dw_model* m, const float* r, int32_t n_walkers, const float* R, const float* Z,
+                        const float* walker_weights, float* out_grad, float* out_log_psi_sqr, dw_stream stream);
+/* Clipped local energies of the loss (optimization/loss_function.py:12-110, one device): clipping center
+ * nanmean (center_median = 0) or nanmedian (1, at most 8192 walkers), width sqrt(nanmean (E - c)^2) (std,
+ * width_mae = 0) or nanmean |E - c| (mae, 1) times clip_by, clip tanh (clip_tanh = 1: c + tanh((E - c) / w)
+ * w) or hard (0: clip to [c - w, c + w]).  With from_previous_step and state_in (device, 2 doubles: center,
+ * width) the previous step's state is used, else it is computed from E.  Writes the per-walker weights
+ * c_b = (E_clip_b - mean E_clip) / B for dw_param_grad, optionally E_clip, and out_stats (device, 8 doubles):
+ * [loss = nanmean E_clip, E_mean, E_var, E_var_clipped, next center, next width, center used, width used];
+ * the next state is computed from E_clip.  FP64, fixed summation order. */
+dw_status dw_clip_energies(const float* e_loc, int32_t n_walkers, int32_t clip_tanh, int32_t center_median,
+                           int32_t width_mae, float clip_by, int32_t from_previous_step, const double* state_in,
+                           float* out_weights, float* out_e_clipped, double* out_stats, dw_stream stream);
+/* Number of floats of the packed parameter gradient. */
+int64_t dw_param_count(const dw_model* m);
+/* Leaf `index` (0 .. number of leaves - 1) of the packed gradient: key "<module suffix>:<leaf>" (e.g.
+ * "h_el_0/linear_0:w", "envelope_orbitals:alpha_up") into key[key_len], its float offset and its shape
+ * (shape[1] = 1 for vectors).  DW_ERR_INVALID past the last leaf. */
+dw_status dw_param_layout(const dw_model* m, int32_t index, char* key, int32_t key_len, int64_t* offset,
+                          int64_t* shape);
+
 /* E_mean and E_var of a batch of local energies in FP64: out_mean_var (device, 2 doubles) receives
  * nanmean(E_loc) and nanmean(|E_loc - E_mean|^2).  NaN walkers are dropped, +-inf kept, as jnp.nanmean
  * does.  Replaces get_observables' energy metrics (optimization/evaluation.py:143-151; pmean over one
