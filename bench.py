@@ -43,10 +43,10 @@ def parse():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3", "bf16"])
     ap.add_argument("--walkers", type=int, default=2048)
     ap.add_argument("--mcmc-steps", type=int, default=20)
-    ap.add_argument("--cpu-walkers", type=int, default=128)
+    ap.add_argument("--cpu-walkers", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--workload", default="n2", choices=["n2", "c4h2-precision", "serving", "batch64", "copy"],
+    ap.add_argument("--workload", default="n2", choices=["n2", "c4h2-precision", "serving", "batch64", "copy", "grad"],
                     help="n2: the headline line; c4h2-precision: the precision-form study (E_loc only); "
                          "serving: Poisson request stream with continuous batching; batch64: 64 geometries "
                          "under shared weights in one graph launch per unit; copy: a plain device copy "
@@ -174,6 +174,95 @@ def copy_calibration(args):
     torch.cuda.synchronize()
     print(json.dumps({"workload": "copy", "bytes_read_per_launch": x.numel() * 4,
                       "bytes_written_per_launch": y.numel() * 4, "launches": 5}))
+
+
+def grad_bench(args):
+    """One optimisation step's gradient on the N2 workload (2048 walkers, 16 dets): E_loc, the clipped
+    statistics and the parameter gradient G = sum_b c_b d log_psi_sqr(r_b) / d theta (the reference's
+    value_and_grad of the clipped total energy, optimization/loss_function.py:75-158), timed per step;
+    the gradient pass alone is timed too.  CPU baseline: the same step (E_loc by 3n HVPs in vmapped chunks,
+    gradient by autograd) in FP32 PyTorch on a walker sample."""
+    import torch
+
+    from deeperwin_amd._native import NativeModel, clip_energies
+    from deeperwin_amd.batching import executed_forward_flops
+    from deeperwin_amd.params import ModelShape, init_params
+    from deeperwin_amd.physics import generate_el_ion_mapping, geometry_n2, init_walkers_gaussian
+
+    R, Z = geometry_n2()
+    n, u, D, B = 14, 7, 16, args.walkers
+    shape = ModelShape(n_el=n, n_up=u, n_ion=2, n_det=D)
+    params = init_params(shape, seed=1)
+    r0 = init_walkers_gaussian(np.random.default_rng(1), B, R, generate_el_ion_mapping(R, Z, n, u))
+    m = NativeModel(shape, args.precision)
+    m.set_params(params)
+    r = torch.as_tensor(r0, dtype=torch.float32, device="cuda").contiguous()
+    Rd = torch.as_tensor(R, dtype=torch.float32, device="cuda")
+    Zd = torch.as_tensor(Z, dtype=torch.float32, device="cuda")
+    e = torch.empty(B, device="cuda")
+    state = torch.tensor([0.0, 1e12], dtype=torch.float64, device="cuda")
+
+    def step():
+        m.local_energy(r, Rd, Zd, e)
+        w, _, stats = clip_energies(e, state)
+        return m.param_grad(r, Rd, Zd, w), w
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g, w = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(args.steps):
+        m.param_grad(r, Rd, Zd, w)
+    b.record()
+    torch.cuda.synchronize()
+    g_ms = a.elapsed_time(b) / args.steps
+    # FLOPs of the gradient pass: one value pass, plus about twice its GEMM work backwards (input and
+    # weight gradients of every GEMM), counted as three value passes
+    gflop = 3 * B * executed_forward_flops(shape) / 1e9
+    peak = PEAK_TFLOPS.get(args.precision, 157.3)
+    out = {"metric": "walker gradient contributions/s (E_loc + clipped loss + parameter gradient)",
+           "value": round(B / dt, 1), "unit": "walkers/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": args.precision, "data": "synthetic",
+           "config": {"workload": "N2, 14 electrons, 16 dets, 256/32 widths, seed 1; one optimisation step's "
+                                  "gradient (E_loc with the full Laplacian, tanh clipping, parameter gradient)",
+                      "walkers": B, "n_params": int(m.lib.dw_param_count(m.handle))},
+           "gradient_pass_ms": round(g_ms, 3),
+           "gradient_pass_tflops": round(gflop / (g_ms / 1e3) / 1e3, 2),
+           "gradient_pass_frac_of_peak": round(gflop / (g_ms / 1e3) / 1e3 / peak, 4)}
+    if not args.no_cpu_baseline:
+        import torch as _t
+
+        from oracle import hamiltonian as oh
+        from oracle import loss as ol
+
+        cores = len(os.sched_getaffinity(0))
+        _t.set_num_threads(cores)
+        k = 64
+        rs = r0[:k].astype(np.float32)
+
+        def cpu_step():
+            ek = oh.local_energy(params, u, n - u, rs, R, Z, dtype=_t.float32, vmapped=True).numpy()
+            ol.loss_gradient(params, u, n - u, rs, R, ek, (0.0, 1e12))
+
+        cpu_step()
+        ts = []
+        for _ in range(3):
+            c0 = time.perf_counter()
+            cpu_step()
+            ts.append(time.perf_counter() - c0)
+        out["cpu_baseline"] = {"value": round(k / float(np.median(ts)), 3), "unit": "walkers/s", "cores": cores,
+                               "kind": "port", "reps": 3,
+                               "sample": f"{k} walkers: E_loc (FP32, vmapped chunk) + clipping + autograd gradient "
+                                         f"(float64 restatement); median of 3 after a warm-up",
+                               "baseline_only": True}
+    print(json.dumps(out))
 
 
 def serving_bench(args):
@@ -434,6 +523,9 @@ def main():
         return
     if args.workload == "copy":
         copy_calibration(args)
+        return
+    if args.workload == "grad":
+        grad_bench(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (walkers shard across GPUs "

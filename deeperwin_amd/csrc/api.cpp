@@ -602,7 +602,7 @@ size_t layout_grad(const dw_model* m, long B, char* base, GradBufs<T>* out) {
   pmax = std::max(pmax, wgrad_part_elems(32, 32, pairs));
   pmax = std::max(pmax, wgrad_part_elems(m->W1, m->NO, B * std::max(m->u, m->dn)));
   g.part = cv.take<T>(pmax);
-  g.cpart = cv.take<T>((size_t)((std::max(pairs, B) + 511) / 512) * std::max(32, m->W1));
+  g.cpart = cv.take<T>((size_t)std::max(colsum_chunks(pairs), colsum_chunks(B)) * std::max(32, m->W1));
   g.epart = cv.take<T>((size_t)((B + kEnvWalkersPerChunk - 1) / kEnvWalkersPerChunk) * 4 * m->I * m->NO);
   if (out) *out = g;
   return cv.off;

@@ -36,7 +36,8 @@ void launch_dz(const T* dX, int ldx, const T* dMU, int ldmu, const T* X, int ldX
 // Per-walker sums over electrons: out[b, o] = sum_i in[b n + i, o]
 template <typename T>
 void launch_walker_sum(const T* in, int ld, int ncols, T* out, const Shape& sh, hipStream_t s);
-// Column sums in fixed order: out[o] (float, scaled) = sum_r in[r, o]; part holds >= chunks * ncols values.
+// Column sums in fixed order: out[o] (float) = sum_r in[r, o]; part holds >= colsum_chunks(rows) * ncols values.
+int colsum_chunks(long rows);
 template <typename T>
 void launch_colsum(const T* in, int ld, long rows, int ncols, T* part, float* out, hipStream_t s);
 

@@ -8,7 +8,9 @@
 // order (chunk partials, then a chunk-ordered sum), so the result does not depend on timing.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
 
 #include "common.h"
 #include "grad.h"
@@ -75,8 +77,6 @@ __global__ __launch_bounds__(64) void k_det_bwd(OrbSrc<T> src, const double* __r
   const int lane = threadIdx.x;
   double* M = lds;                 // [n][2n]
   double* envs = lds + n * n2;     // [n][n] envelope of element (i, k)
-  __shared__ int piv;
-  __shared__ double pv;
   // form A (rows i = electrons, columns k = orbitals of det d) and the identity block
   for (int t = lane; t < n * n; t += 64) {
     const int i = t / n, k = t - (t / n) * n;
@@ -96,22 +96,25 @@ __global__ __launch_bounds__(64) void k_det_bwd(OrbSrc<T> src, const double* __r
     envs[i * n + k] = (double)env;
   }
   __syncthreads();
+  __shared__ double fcol[64];
   for (int p = 0; p < n; ++p) {
-    // pivot: the row r >= p with the largest |M[r][p]| (first on ties), found by one lane
-    if (lane == 0) {
-      int best = p;
-      double bv = fabs(M[p * n2 + p]);
-      for (int rr = p + 1; rr < n; ++rr) {
-        const double v = fabs(M[rr * n2 + p]);
-        if (v > bv) {
-          bv = v;
-          best = rr;
-        }
-      }
-      piv = best;
+    // pivot: the row r >= p with the largest |M[r][p]| (the first of equal ones), lanes over rows
+    double v = -1.0;  // lanes outside [p, n) never win; a NaN entry counts as the largest
+    if (lane >= p && lane < n) {
+      const double a = fabs(M[lane * n2 + p]);
+      v = isnan(a) ? 1.0e308 : a;
     }
-    __syncthreads();
-    const int pr = piv;
+    int idx = lane;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double v2 = __shfl_xor(v, o, 64);
+      const int i2 = __shfl_xor(idx, o, 64);
+      if (v2 > v || (v2 == v && i2 < idx)) {
+        v = v2;
+        idx = i2;
+      }
+    }
+    const int pr = (idx >= p && idx < n) ? idx : p;
     if (pr != p)
       for (int j = lane; j < n2; j += 64) {
         const double t0 = M[p * n2 + j];
@@ -119,17 +122,15 @@ __global__ __launch_bounds__(64) void k_det_bwd(OrbSrc<T> src, const double* __r
         M[pr * n2 + j] = t0;
       }
     __syncthreads();
-    if (lane == 0) pv = 1.0 / M[p * n2 + p];
+    const double inv = 1.0 / M[p * n2 + p];
+    if (lane < n) fcol[lane] = M[lane * n2 + p];  // factors of every row, read before any row changes
     __syncthreads();
-    const double inv = pv;
     for (int j = lane; j < n2; j += 64) M[p * n2 + j] *= inv;
     __syncthreads();
-    // eliminate column p from every other row; lanes over columns, each reads its factor first
-    for (int rr = 0; rr < n; ++rr) {
-      if (rr == p) continue;
-      const double f = M[rr * n2 + p];
-      __syncthreads();
-      for (int j = lane; j < n2; j += 64) M[rr * n2 + j] -= f * M[p * n2 + j];
+    // eliminate column p from every other row: lanes over all (row, column) elements
+    for (int t = lane; t < n * n2; t += 64) {
+      const int rr = t / n2, j = t - (t / n2) * n2;
+      if (rr != p) M[rr * n2 + j] -= fcol[rr] * M[p * n2 + j];
     }
     __syncthreads();
   }
@@ -233,17 +234,35 @@ __global__ void k_walker_sum(const T* __restrict__ in, int ld, int ncols, T* __r
   out[b * ncols + o] = acc;
 }
 
-constexpr int kColChunk = 512;
+}  // namespace
 
+// Row chunks of a column sum: about 128 chunks of at least 256 rows.
+int colsum_chunks(long rows) { return (int)std::min<long>(128, std::max<long>(1, (rows + 255) / 256)); }
+
+namespace {
+
+// Column sums of a chunk of rows: 32 columns x 8 row lanes per 256-thread block; the 8 row lanes meet in
+// LDS in a fixed order.
 template <typename T>
-__global__ void k_colsum_part(const T* __restrict__ in, int ld, long rows, int ncols, T* __restrict__ part) {
-  const int o = blockIdx.x * blockDim.x + threadIdx.x;
-  const long r0 = (long)blockIdx.y * kColChunk;
-  if (o >= ncols) return;
-  const long r1 = r0 + kColChunk < rows ? r0 + kColChunk : rows;
+__global__ __launch_bounds__(256) void k_colsum_part(const T* __restrict__ in, int ld, long rows, int ncols,
+                                                     T* __restrict__ part) {
+  __shared__ T red[8][32];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int o = blockIdx.x * 32 + tx;
+  const long per = (rows + gridDim.y - 1) / gridDim.y;
+  const long r0 = (long)blockIdx.y * per;
+  const long r1 = r0 + per < rows ? r0 + per : rows;
   T acc = T(0);
-  for (long r = r0; r < r1; ++r) acc += in[r * ld + o];
-  part[(long)blockIdx.y * ncols + o] = acc;
+  if (o < ncols)
+    for (long r = r0 + ty; r < r1; r += 8) acc += in[r * ld + o];
+  red[ty][tx] = acc;
+  __syncthreads();
+  if (ty == 0 && o < ncols) {
+    T t = red[0][tx];
+#pragma unroll
+    for (int q = 1; q < 8; ++q) t += red[q][tx];
+    part[(long)blockIdx.y * ncols + o] = t;
+  }
 }
 
 template <typename T>
@@ -349,40 +368,66 @@ __global__ __launch_bounds__(128) void k_twoel_bwd(const float* __restrict__ r, 
 }
 
 // ---- weight gradients -------------------------------------------------------------------------------
-// C[m, n] = sum_k A[k, m] B[k, n].  64 x 64 output tiles per 256-thread workgroup; the K rows are split in
-// chunks of kWgradChunk, each chunk's tile goes to its own partial buffer, and k_sum_parts adds the chunks
-// in order.  FP32: v_mfma_f32_32x32x2_f32 (4 waves of 32 x 32); FP64: 4 x 4 outputs per thread on FMA.
-constexpr int kWgradChunk = 1024;
+// C[m, n] = sum_k A[k, m] B[k, n].  The K rows are split into chunks (about 1024 workgroups in all); each
+// chunk's tile goes to its own partial buffer, and k_sum_parts_ld adds the chunks in order.
+// FP32, M or N > 32: 64 x 64 tiles per 256-thread workgroup, v_mfma_f32_32x32x2_f32 (4 waves of 32 x 32),
+// K slices of 16 rows staged through LDS, the next slice's loads issued before the current slice's products.
+// FP32, M and N <= 32 (2-el layers): one 32 x 32 block, no LDS staging: each wave streams every fourth
+// 16-row slice of the chunk straight into MFMA operands (lane l takes row l / 32, column l % 32, which
+// are coalesced 128-byte rows), and the four waves' accumulators meet in LDS in a fixed order.
+// FP64: 64 x 64 tiles, 4 x 4 outputs per thread on FMA.
+constexpr int kWgradBK = 16;
 
 __global__ __launch_bounds__(256) void k_wgrad_f32(const float* __restrict__ A, int lda, const float* __restrict__ Bm,
-                                                   int ldb, int M, int N, long K, float* __restrict__ part,
-                                                   int tilesN) {
-  constexpr int BM = 64, BN = 64, BK = 16;
+                                                   int ldb, int M, int N, long K, long kchunk,
+                                                   float* __restrict__ part, int tilesN) {
+  constexpr int BM = 64, BN = 64, BK = kWgradBK;
   __shared__ float As[BK][BM + 4];
   __shared__ float Bs[BK][BN + 4];
   const int tile = blockIdx.x, chunk = blockIdx.y;
   const int m0 = (tile / tilesN) * BM, n0 = (tile % tilesN) * BN;
-  const long k0 = (long)chunk * kWgradChunk;
-  const long k1 = k0 + kWgradChunk < K ? k0 + kWgradChunk : K;
+  const long k0 = (long)chunk * kchunk;
+  const long k1 = k0 + kchunk < K ? k0 + kchunk : K;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int wm = wid >> 1, wn = wid & 1;
+  // one float4 of A and of B per thread and slice when the tile is full and aligned
+  const bool vec = ((lda & 3) == 0) && ((ldb & 3) == 0) && ((((uintptr_t)A) & 15) == 0) &&
+                   ((((uintptr_t)Bm) & 15) == 0) && m0 + BM <= M && n0 + BN <= N;
+  const int vr = tid >> 4, vc = (tid & 15) * 4;
   v16f acc;
   for (int j = 0; j < 16; ++j) acc[j] = 0.f;
+  float4 ra = make_float4(0.f, 0.f, 0.f, 0.f), rb = ra;
+  auto gload = [&](long kb) {
+    const long k = kb + vr;
+    if (k < k1) {
+      ra = *(const float4*)(A + k * lda + m0 + vc);
+      rb = *(const float4*)(Bm + k * ldb + n0 + vc);
+    } else {
+      ra = make_float4(0.f, 0.f, 0.f, 0.f);
+      rb = ra;
+    }
+  };
+  if (vec) gload(k0);
   for (long kb = k0; kb < k1; kb += BK) {
     __syncthreads();
-    for (int t = tid; t < BK * BM; t += 256) {
-      const int kk = t / BM, mm = t - (t / BM) * BM;
-      const long k = kb + kk;
-      As[kk][mm] = (k < k1 && m0 + mm < M) ? A[k * lda + m0 + mm] : 0.f;
-      Bs[kk][mm] = (k < k1 && n0 + mm < N) ? Bm[k * ldb + n0 + mm] : 0.f;
+    if (vec) {
+      *(float4*)&As[vr][vc] = ra;
+      *(float4*)&Bs[vr][vc] = rb;
+    } else {
+      for (int t = tid; t < BK * BM; t += 256) {
+        const int kk = t / BM, mm = t - (t / BM) * BM;
+        const long k = kb + kk;
+        As[kk][mm] = (k < k1 && m0 + mm < M) ? A[k * lda + m0 + mm] : 0.f;
+        Bs[kk][mm] = (k < k1 && n0 + mm < N) ? Bm[k * ldb + n0 + mm] : 0.f;
+      }
     }
     __syncthreads();
+    if (vec && kb + BK < k1) gload(kb + BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 2) {
       const int kr = kk + (lane >> 5);
-      const float av = As[kr][wm * 32 + (lane & 31)];
-      const float bv = Bs[kr][wn * 32 + (lane & 31)];
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(As[kr][wm * 32 + (lane & 31)], Bs[kr][wn * 32 + (lane & 31)], acc,
+                                                 0, 0, 0);
     }
   }
   float* P = part + (long)chunk * M * N;
@@ -394,16 +439,53 @@ __global__ __launch_bounds__(256) void k_wgrad_f32(const float* __restrict__ A, 
   }
 }
 
+__global__ __launch_bounds__(256) void k_wgrad_small_f32(const float* __restrict__ A, int lda,
+                                                         const float* __restrict__ Bm, int ldb, int M, int N, long K,
+                                                         long kchunk, float* __restrict__ part) {
+  __shared__ float red[4][16][64];
+  const int chunk = blockIdx.x;
+  const long k0 = (long)chunk * kchunk;
+  const long k1 = k0 + kchunk < K ? k0 + kchunk : K;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int col = lane & 31, half = lane >> 5;
+  const bool am = col < M, bn = col < N;
+  v16f acc;
+  for (int j = 0; j < 16; ++j) acc[j] = 0.f;
+  for (long kb = k0 + (long)wid * kWgradBK; kb < k1; kb += 4L * kWgradBK) {
+    float a[kWgradBK / 2], b[kWgradBK / 2];
+#pragma unroll
+    for (int s = 0; s < kWgradBK / 2; ++s) {
+      const long k = kb + 2 * s + half;
+      a[s] = (am && k < k1) ? A[k * lda + col] : 0.f;
+      b[s] = (bn && k < k1) ? Bm[k * ldb + col] : 0.f;
+    }
+#pragma unroll
+    for (int s = 0; s < kWgradBK / 2; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[s], b[s], acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int j = 0; j < 16; ++j) red[wid][j][lane] = acc[j];
+  __syncthreads();
+  if (wid == 0) {
+    float* P = part + (long)chunk * M * N;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const float v = ((red[0][j][lane] + red[1][j][lane]) + red[2][j][lane]) + red[3][j][lane];
+      const int mm = 8 * (j >> 2) + 4 * half + (j & 3);
+      if (mm < M && col < N) P[(long)mm * N + col] = v;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_wgrad_f64(const double* __restrict__ A, int lda, const double* __restrict__ Bm,
-                                                   int ldb, int M, int N, long K, double* __restrict__ part,
-                                                   int tilesN) {
+                                                   int ldb, int M, int N, long K, long kchunk,
+                                                   double* __restrict__ part, int tilesN) {
   constexpr int BM = 64, BN = 64, BK = 16;
   __shared__ double As[BK][BM + 1];
   __shared__ double Bs[BK][BN + 1];
   const int tile = blockIdx.x, chunk = blockIdx.y;
   const int m0 = (tile / tilesN) * BM, n0 = (tile % tilesN) * BN;
-  const long k0 = (long)chunk * kWgradChunk;
-  const long k1 = k0 + kWgradChunk < K ? k0 + kWgradChunk : K;
+  const long k0 = (long)chunk * kchunk;
+  const long k1 = k0 + kchunk < K ? k0 + kchunk : K;
   const int tid = threadIdx.x;
   const int tm = (tid >> 4) * 4, tn = (tid & 15) * 4;
   double acc[4][4] = {};
@@ -482,8 +564,8 @@ void launch_walker_sum(const T* in, int ld, int ncols, T* out, const Shape& sh, 
 
 template <typename T>
 void launch_colsum(const T* in, int ld, long rows, int ncols, T* part, float* out, hipStream_t s) {
-  const int chunks = (int)((rows + kColChunk - 1) / kColChunk);
-  hipLaunchKernelGGL(k_colsum_part<T>, dim3((ncols + 63) / 64, chunks), dim3(64), 0, s, in, ld, rows, ncols, part);
+  const int chunks = colsum_chunks(rows);
+  hipLaunchKernelGGL(k_colsum_part<T>, dim3((ncols + 31) / 32, chunks), dim3(256), 0, s, in, ld, rows, ncols, part);
   hipLaunchKernelGGL(k_sum_parts<T>, dim3((ncols + 255) / 256), dim3(256), 0, s, part, chunks, (long)ncols, out);
 }
 
@@ -511,17 +593,40 @@ void launch_twoel_bwd(const float* r, T* const* dX, const int* Kx, const int* Kh
   hipLaunchKernelGGL(k_twoel_bwd<T>, dim3((unsigned)((pairs + 127) / 128)), dim3(128), 0, s, r, a, sh);
 }
 
+// Chunks of the K rows: about 1024 workgroups in all, at least 64 rows per chunk, at most 256 chunks.
+static void wgrad_split(int M, int N, long K, int& chunks, long& kchunk) {
+  const bool small = M <= 32 && N <= 32;
+  const long tiles = small ? 1 : (long)((M + 63) / 64) * ((N + 63) / 64);
+  long c = (1024 + tiles - 1) / tiles;
+  c = std::min<long>(c, 256);
+  c = std::min<long>(c, (K + 63) / 64);
+  c = std::max<long>(c, 1);
+  kchunk = (K + c - 1) / c;
+  kchunk = (kchunk + kWgradBK - 1) / kWgradBK * kWgradBK;
+  chunks = (int)((K + kchunk - 1) / kchunk);
+}
+
 size_t wgrad_part_elems(int M, int N, long K) {
-  return (size_t)((K + kWgradChunk - 1) / kWgradChunk) * (size_t)M * (size_t)N;
+  int chunks;
+  long kchunk;
+  wgrad_split(M, N, K, chunks, kchunk);
+  return (size_t)chunks * (size_t)M * (size_t)N;
 }
 
 template <>
 void wgrad<float>(const float* A, int lda, const float* Bm, int ldb, int M, int N, long K, float* part, float* C,
                   int ldc, hipStream_t s) {
   if (M <= 0 || N <= 0 || K <= 0) return;
-  const int chunks = (int)((K + kWgradChunk - 1) / kWgradChunk);
-  const int tilesN = (N + 63) / 64, tiles = ((M + 63) / 64) * tilesN;
-  hipLaunchKernelGGL(k_wgrad_f32, dim3(tiles, chunks), dim3(256), 0, s, A, lda, Bm, ldb, M, N, K, part, tilesN);
+  int chunks;
+  long kchunk;
+  wgrad_split(M, N, K, chunks, kchunk);
+  if (M <= 32 && N <= 32) {
+    hipLaunchKernelGGL(k_wgrad_small_f32, dim3(chunks), dim3(256), 0, s, A, lda, Bm, ldb, M, N, K, kchunk, part);
+  } else {
+    const int tilesN = (N + 63) / 64, tiles = ((M + 63) / 64) * tilesN;
+    hipLaunchKernelGGL(k_wgrad_f32, dim3(tiles, chunks), dim3(256), 0, s, A, lda, Bm, ldb, M, N, K, kchunk, part,
+                       tilesN);
+  }
   hipLaunchKernelGGL(k_sum_parts_ld<float>, dim3((unsigned)(((long)M * N + 255) / 256)), dim3(256), 0, s, part, chunks,
                      M, N, C, ldc);
 }
@@ -530,9 +635,11 @@ template <>
 void wgrad<double>(const double* A, int lda, const double* Bm, int ldb, int M, int N, long K, double* part, float* C,
                    int ldc, hipStream_t s) {
   if (M <= 0 || N <= 0 || K <= 0) return;
-  const int chunks = (int)((K + kWgradChunk - 1) / kWgradChunk);
+  int chunks;
+  long kchunk;
+  wgrad_split(M, N, K, chunks, kchunk);
   const int tilesN = (N + 63) / 64, tiles = ((M + 63) / 64) * tilesN;
-  hipLaunchKernelGGL(k_wgrad_f64, dim3(tiles, chunks), dim3(256), 0, s, A, lda, Bm, ldb, M, N, K, part, tilesN);
+  hipLaunchKernelGGL(k_wgrad_f64, dim3(tiles, chunks), dim3(256), 0, s, A, lda, Bm, ldb, M, N, K, kchunk, part, tilesN);
   hipLaunchKernelGGL(k_sum_parts_ld<double>, dim3((unsigned)(((long)M * N + 255) / 256)), dim3(256), 0, s, part, chunks,
                      M, N, C, ldc);
 }

@@ -76,4 +76,26 @@ PINNING = {
     "mcmc.adjust_stepsize": [
         "test_oracle_mcmc.py::test_stepsize_clip_and_increase",
     ],
+    "hamiltonian.kinetic_energy_vmapped": [
+        "test_oracle_physics.py::test_vmapped_kinetic_energy_equals_walker_loop (against the pinned walker loop)",
+    ],
+    "loss.clipping_center_and_width": [
+        "test_oracle_loss.py::test_clipping_center_and_width_known_answers (hand-worked mean, median, std, mae)",
+    ],
+    "loss.clip": [
+        "test_oracle_loss.py::test_clip_functions_known_answers (hard and tanh clips, NaN kept)",
+    ],
+    "loss.clip_energies": [
+        "test_oracle_loss.py::test_clip_functions_known_answers (initial state (0, 1e12) leaves E unchanged)",
+    ],
+    "loss.total_energy": [
+        "test_oracle_loss.py::test_clip_functions_known_answers (statistics of the clipped energies)",
+    ],
+    "loss.param_grad": [
+        "test_oracle_loss.py::test_param_grad_matches_finite_differences (every leaf kind, central differences)",
+    ],
+    "loss.loss_gradient": [
+        "test_oracle_loss.py::test_clip_functions_known_answers (its statistics: loss.total_energy)",
+        "test_oracle_loss.py::test_param_grad_matches_finite_differences (its gradient: loss.param_grad)",
+    ],
 }
