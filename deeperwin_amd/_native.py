@@ -56,6 +56,9 @@ SIGNATURES = {
     "dw_mcmc_steps_noalloc": ([_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _f32, _f32, _f32,
                                _vp, _vp, _vp], ctypes.c_int),
     "dw_energy_moments": ([_vp, _i32, _vp, _vp], ctypes.c_int),
+    "dw_threefry_split": ([_vp, _i32, _vp, _vp], ctypes.c_int),
+    "dw_threefry_normal": ([_vp, ctypes.c_int64, _vp, _vp], ctypes.c_int),
+    "dw_threefry_mcmc_draws": ([_vp, _vp, _i32, _i32, _i32, _vp, _vp, _vp], ctypes.c_int),
     "dw_param_grad": ([_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
     "dw_param_count": ([_vp], ctypes.c_int64),
     "dw_clip_energies": ([_vp, _i32, _i32, _i32, _i32, _f32, _i32, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),

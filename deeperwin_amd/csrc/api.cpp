@@ -1089,6 +1089,32 @@ dw_status dw_param_grad(dw_model* m, const float* r, int32_t n_walkers, const fl
   return run_grad<float>(m, r, n_walkers, R, Z, walker_weights, out_grad, out_log_psi_sqr, s);
 }
 
+dw_status dw_threefry_split(const uint32_t* key, int32_t num, uint32_t* out, dw_stream stream) {
+  if (!key || !out) return fail(DW_ERR_INVALID, "null argument");
+  if (num <= 0) return fail(DW_ERR_INVALID, "num must be positive");
+  launch_threefry_split(key, num, out, (hipStream_t)stream);
+  HIP_TRY(hipGetLastError());
+  return DW_OK;
+}
+
+dw_status dw_threefry_normal(const uint32_t* key, int64_t count, float* out, dw_stream stream) {
+  if (!key || !out) return fail(DW_ERR_INVALID, "null argument");
+  if (count <= 0 || count > 0xFFFFFFFFLL) return fail(DW_ERR_INVALID, "count must be in [1, 2^32)");
+  launch_threefry_normal(key, (long)count, out, (hipStream_t)stream);
+  HIP_TRY(hipGetLastError());
+  return DW_OK;
+}
+
+dw_status dw_threefry_mcmc_draws(const uint32_t* keys_in, uint32_t* keys_out, int32_t n_walkers, int32_t n_el,
+                                 int32_t n_steps, float* normals, float* uniforms, dw_stream stream) {
+  if (!keys_in || !keys_out || !normals || !uniforms) return fail(DW_ERR_INVALID, "null argument");
+  if (keys_in == keys_out) return fail(DW_ERR_INVALID, "keys_out must not alias keys_in");
+  if (n_walkers <= 0 || n_el <= 0 || n_steps <= 0) return fail(DW_ERR_INVALID, "bad sizes");
+  launch_threefry_mcmc(keys_in, keys_out, n_walkers, n_el, n_steps, normals, uniforms, (hipStream_t)stream);
+  HIP_TRY(hipGetLastError());
+  return DW_OK;
+}
+
 dw_status dw_energy_moments(const float* e_loc, int32_t n_walkers, double* out_mean_var, dw_stream stream) {
   if (!e_loc || !out_mean_var) return fail(DW_ERR_INVALID, "null argument");
   if (n_walkers <= 0) return fail(DW_ERR_INVALID, "n_walkers must be positive");

@@ -91,6 +91,11 @@ void launch_combine(const double* ld, const double* sd, const double* g, const d
                     float* out_eloc, uint32_t* out_flags, float* out_kappa, const Shape& sh, hipStream_t s);
 
 void launch_fill_i32(int32_t* p, int32_t v, int count, hipStream_t s);
+// JAX threefry streams (threefry.hip)
+void launch_threefry_split(const uint32_t* key, int num, uint32_t* out, hipStream_t s);
+void launch_threefry_normal(const uint32_t* key, long count, float* out, hipStream_t s);
+void launch_threefry_mcmc(const uint32_t* keys_in, uint32_t* keys_out, int B, int n, int n_steps, float* normals,
+                          float* uniforms, hipStream_t s);
 // out[0] = nanmean(e[0:n]), out[1] = nanmean(|e - out[0]|^2), FP64, deterministic order.
 void launch_energy_moments(const float* e, int n, double* out, hipStream_t s);
 // Clipped-energy statistics and per-walker gradient weights (optimization/loss_function.py:12-110); one

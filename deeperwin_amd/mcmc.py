@@ -7,9 +7,12 @@ log|psi|^2 of the proposals, accept/reject, walker ages, the running acceptance 
 schedule (stepsize, acc_rate and step_nr live in a 16-byte device record).
 
 Random draws: the reference derives proposals and accept thresholds from per-walker threefry keys
-(mcmc.py:178, 360-361).  Here they are drawn on the device from a seeded ``torch.Generator`` held in
-``state.rng_state`` — or supplied explicitly through ``draws=(normals[S, B, n, 3], uniforms[S, B])`` so
-that a chain can be replayed exactly by the CPU oracle.
+(mcmc.py:178, 360-361).  When ``state.rng_state`` holds such keys (a CUDA int32 tensor [B, 2] of uint32
+words, as ``initialize_around_nuclei`` makes from a PRNG key), the draws are the reference's own streams,
+generated on the device by ``deeperwin_amd.threefry`` (jax 0.4.23's threefry2x32; the proposal normal and
+the accept uniform of a step share one subkey, as in the reference).  A ``torch.Generator`` in
+``rng_state`` draws instead from that generator; ``draws=(normals[S, B, n, 3], uniforms[S, B])`` supplies
+them explicitly so that a chain can be replayed exactly by the CPU oracle.
 """
 from __future__ import annotations
 
@@ -68,12 +71,30 @@ class MCMCState:
     def initialize_around_nuclei(cls, n_walkers: int, physical_config: PhysicalConfig, init_method: str = "gaussian",
                                  spin_initialization: str = "el_ion_mapping", rng=0, stepsize: float = 1e-2):
         """Gaussian initialisation around the mapped nuclei (mcmc.py:40-91, branch 64-67); log_psi_sqr
-        starts at -1000 so the first move is always accepted; ages start at 0."""
+        starts at -1000 so the first move is always accepted; ages start at 0.
+
+        With a PRNG key for `rng` (2 uint32 words: ``threefry.PRNGKey(seed)``, a numpy uint32 pair or a
+        CUDA int32 tensor) this follows the reference step by step: rng_r, rng_spin, rng = split(rng, 3);
+        r0 = R[mapping] + normal(rng_r, [B, n, 3]); the walker keys are split(rng, B).  With an integer
+        seed, r0 comes from numpy's generator and later draws from a seeded torch.Generator."""
         if init_method != "gaussian":
             raise NotImplementedError("only the Gaussian walker initialisation is on the native path")
         if spin_initialization != "el_ion_mapping":
             raise NotImplementedError("only spin_initialization='el_ion_mapping' is on the native path")
-        seed = rng if isinstance(rng, (int, np.integer)) else int(np.random.default_rng(rng).integers(2**31))
+        if not isinstance(rng, (int, np.integer)):
+            from . import threefry
+
+            rng_r, _rng_spin, rng_k = threefry.split(rng, 3)
+            n = physical_config.n_electrons
+            offset = torch.as_tensor(np.asarray(physical_config.R, np.float32)[np.asarray(
+                physical_config.el_ion_mapping)], device="cuda")
+            r0 = (threefry.normal(rng_r, (n_walkers, n, 3)) + offset).contiguous()
+            return cls(r=r0, R=_as_device(physical_config.R),
+                       Z=_as_device(np.asarray(physical_config.Z, dtype=np.float32)),
+                       log_psi_sqr=torch.full((n_walkers,), -1000.0, device="cuda"),
+                       walker_age=torch.zeros(n_walkers, dtype=torch.int32, device="cuda"),
+                       rng_state=threefry.split(rng_k, n_walkers), chain=make_chain(stepsize))
+        seed = int(rng)
         gen = np.random.default_rng(seed)
         r0 = init_walkers_gaussian(gen, n_walkers, physical_config.R, physical_config.el_ion_mapping)
         tg = torch.Generator(device="cuda")
@@ -91,11 +112,16 @@ class MetropolisHastingsMonteCarlo:
         self.config = mcmc_config
 
     def draw(self, state: MCMCState, n_steps: int):
+        """(normals, uniforms, next rng_state) for n_steps steps of the walkers in `state`."""
         B, n, _ = state.r.shape
         g = state.rng_state
+        if isinstance(g, torch.Tensor):  # per-walker threefry keys: the reference's own streams
+            from . import threefry
+
+            return threefry.mcmc_draws(g, n, n_steps)
         normals = torch.randn((n_steps, B, n, 3), generator=g, device="cuda", dtype=torch.float32)
         uniforms = torch.rand((n_steps, B), generator=g, device="cuda", dtype=torch.float32)
-        return normals, uniforms
+        return normals, uniforms, g
 
     def run_mcmc_steps(self, func, state: MCMCState, params, n_up: int, n_dn: int, fixed_params, n_steps: int,
                        draws=None) -> MCMCState:
@@ -111,7 +137,10 @@ class MetropolisHastingsMonteCarlo:
                           else torch.zeros(B, dtype=torch.int32, device="cuda"))
         new.chain = state.chain.clone() if state.chain is not None else make_chain(1e-2)
         if n_steps > 0:
-            normals, uniforms = draws if draws is not None else self.draw(state, n_steps)
+            if draws is not None:
+                normals, uniforms = draws
+            else:
+                normals, uniforms, new.rng_state = self.draw(state, n_steps)
             normals = _as_device(normals)
             uniforms = _as_device(uniforms)
             if normals.shape != (n_steps, B, n_up + n_dn, 3) or uniforms.shape != (n_steps, B):

@@ -153,6 +153,20 @@ int64_t dw_param_count(const dw_model* m);
 dw_status dw_param_layout(const dw_model* m, int32_t index, char* key, int32_t key_len, int64_t* offset,
                           int64_t* shape);
 
+/* ---- JAX random streams (jax 0.4.23's default threefry2x32 PRNG; keys are 2 uint32 words on the device) ----
+ * so that the sampler replays the reference's draws (mcmc.py:43, 90, 175-180, 345-378).
+ * dw_threefry_split: jax.random.split(key, num) -> out [num, 2].
+ * dw_threefry_normal: jax.random.normal(key, shape) for `count` = prod(shape) float32 values (row-major).
+ * dw_threefry_mcmc_draws: the sampler's per-walker stream for n_steps steps: at each step the walker key k
+ *   gives s = split(k)[1] and the next key split(k)[0]; normals[step, b] = normal(s, (n_el, 3)) (the
+ *   proposal, mcmc.py:178) and uniforms[step, b] = uniform(s, ()) (the accept threshold, mcmc.py:360-361,
+ *   the same subkey).  keys_in [B, 2] -> keys_out [B, 2] (must not alias).  The normals' erfinv is the
+ *   single-precision polynomial of M. Giles (GPU Computing Gems 2011), as XLA's; the uniforms are exact. */
+dw_status dw_threefry_split(const uint32_t* key, int32_t num, uint32_t* out, dw_stream stream);
+dw_status dw_threefry_normal(const uint32_t* key, int64_t count, float* out, dw_stream stream);
+dw_status dw_threefry_mcmc_draws(const uint32_t* keys_in, uint32_t* keys_out, int32_t n_walkers, int32_t n_el,
+                                 int32_t n_steps, float* normals, float* uniforms, dw_stream stream);
+
 /* E_mean and E_var of a batch of local energies in FP64: out_mean_var (device, 2 doubles) receives
  * nanmean(E_loc) and nanmean(|E_loc - E_mean|^2).  NaN walkers are dropped, +-inf kept, as jnp.nanmean
  * does.  Replaces get_observables' energy metrics (optimization/evaluation.py:143-151; pmean over one

@@ -79,6 +79,32 @@ PINNING = {
     "hamiltonian.kinetic_energy_vmapped": [
         "test_oracle_physics.py::test_vmapped_kinetic_energy_equals_walker_loop (against the pinned walker loop)",
     ],
+    "threefry.threefry2x32": [
+        "test_oracle_threefry.py::test_threefry_random123_vectors (Random123 known-answer vectors)",
+    ],
+    "threefry.split": [
+        "test_oracle_threefry.py::test_jax_documented_values (split(PRNGKey(0)) as JAX's documentation prints it)",
+    ],
+    "threefry.random_bits": [
+        "test_oracle_threefry.py::test_jax_documented_values (normal(PRNGKey(0), (10,)), normal(PRNGKey(42)))",
+    ],
+    "threefry.uniform": [
+        "test_oracle_threefry.py::test_erfinv32_and_uniform_fill (mantissa fill of hand-picked words)",
+        "test_oracle_threefry.py::test_jax_documented_values",
+    ],
+    "threefry.erfinv32": [
+        "test_oracle_threefry.py::test_erfinv32_and_uniform_fill (scipy's float64 erfinv)",
+    ],
+    "threefry.normal": [
+        "test_oracle_threefry.py::test_jax_documented_values",
+    ],
+    "threefry.mcmc_draws": [
+        "test_oracle_threefry.py::test_mcmc_stream_shares_one_subkey_per_step (key chain and shared subkey)",
+    ],
+    "threefry.init_walkers": [
+        "test_oracle_threefry.py::test_jax_documented_values (its split and normal; the composition follows "
+        "mcmc.py:42-66, 90)",
+    ],
     "loss.clipping_center_and_width": [
         "test_oracle_loss.py::test_clipping_center_and_width_known_answers (hand-worked mean, median, std, mae)",
     ],
