@@ -21,6 +21,7 @@ DW_FP32, DW_BF16X3, DW_BF16, DW_FP64 = 0, 1, 2, 3
 PRECISIONS = {"fp32": DW_FP32, "bf16x3": DW_BF16X3, "bf16": DW_BF16, "fp64": DW_FP64}
 FLAG_NONFINITE, FLAG_FP64_DET, FLAG_FORCED_ACCEPT = 1, 2, 4
 TIMER_GEMM_1EL_LAP, TIMER_DET_LAP, TIMER_FORWARD, TIMER_LOCAL_ENERGY = 0, 1, 2, 3
+PROPOSALS = {"normal": 0, "local": 1, "langevin": 2}
 
 # Exported symbols and their ctypes signatures (argtypes, restype).
 _vp, _i32, _f32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_float
@@ -55,6 +56,8 @@ SIGNATURES = {
                        _vp, _vp], ctypes.c_int),
     "dw_mcmc_steps_noalloc": ([_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _f32, _f32, _f32,
                                _vp, _vp, _vp], ctypes.c_int),
+    "dw_model_set_proposal": ([_vp, _i32, _f32, _f32, _f32], ctypes.c_int),
+    "dw_mcmc_propose": ([_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
     "dw_energy_moments": ([_vp, _i32, _vp, _vp], ctypes.c_int),
     "dw_threefry_split": ([_vp, _i32, _vp, _vp], ctypes.c_int),
     "dw_threefry_normal": ([_vp, ctypes.c_int64, _vp, _vp], ctypes.c_int),
@@ -275,6 +278,18 @@ class NativeModel:
     def local_energy(self, r, R, Z, out_eloc, out_lp=None, out_flags=None) -> None:
         _check(self.lib.dw_local_energy(self.handle, _ptr(r), r.shape[0], _ptr(R), _ptr(Z), _ptr(out_eloc),
                                         _ptr(out_lp), _ptr(out_flags), _stream()))
+
+    def set_proposal(self, name: str = "normal", r_min: float = 0.1, r_max: float = 1.0,
+                     langevin_scale: float = 1.0) -> None:
+        """dw_model_set_proposal: 'normal', 'local' or 'langevin' (the proposal names of MCMCConfig)."""
+        if name not in PROPOSALS:
+            raise ValueError(f"unknown proposal {name!r}; the device sampler has {sorted(PROPOSALS)}")
+        _check(self.lib.dw_model_set_proposal(self.handle, PROPOSALS[name], float(r_min), float(r_max),
+                                              float(langevin_scale)))
+
+    def propose(self, r, R, Z, normals, chain, out_r, out_log_q) -> None:
+        _check(self.lib.dw_mcmc_propose(self.handle, _ptr(r), r.shape[0], _ptr(R), _ptr(Z), _ptr(normals),
+                                        _ptr(chain), _ptr(out_r), _ptr(out_log_q), _stream()))
 
     def mcmc_steps(self, r, R, Z, lp, age, normals, uniforms, chain, max_age, interval, target, lo, hi,
                    out_acc=None, out_flags=None, noalloc: bool = False) -> None:

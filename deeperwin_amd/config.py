@@ -7,15 +7,16 @@ the reference's pydantic defaults for the FermiNet-style model:
 * FermiNet embedding: 4 iterations, 256 / 32 widths    (configuration.py:277-375)
 * FermiNet input features                              (configuration.py:634-652)
 * envelope orbitals, full determinant, 32 dets         (configuration.py:688-699, 783-806)
-* MCMC: Gaussian proposal, target acceptance 0.5,
-  stepsize clip [1e-2, 1.0], evaluation max_age 100    (configuration.py:978-1050)
+* MCMC: Gaussian proposal (or the local-stepsize and
+  Langevin proposals), target acceptance 0.5,
+  stepsize clip [1e-2, 1.0], evaluation max_age 100    (configuration.py:952-1050)
 * local energy: chunks of 64 walkers, forward_lap off  (configuration.py:1264-1266)
 """
 from __future__ import annotations
 
 import dataclasses
 from dataclasses import dataclass, field
-from typing import List, Optional
+from typing import List, Optional, Union
 
 
 @dataclass
@@ -95,6 +96,23 @@ class MCMCSimpleProposalConfig:
 
 
 @dataclass
+class MCMCLangevinProposalConfig:
+    """Local stepsize plus a drift towards the nuclei (configuration.py:956-963)."""
+    name: str = "langevin"
+    langevin_scale: float = 1.0
+    r_min: float = 0.2
+    r_max: float = 2.0
+
+
+@dataclass
+class LocalStepsizeProposalConfig:
+    """Stepsize scaled by the distance to the closest nucleus (configuration.py:966-975)."""
+    name: str = "local"
+    r_min: float = 0.1
+    r_max: float = 1.0
+
+
+@dataclass
 class MCMCConfig:
     """Evaluation defaults (configuration.py:1045-1049); the configs fix the Gaussian init and
     stepsize 0.02 bohr instead of the reference's exponential init and 1e-2."""
@@ -108,14 +126,16 @@ class MCMCConfig:
     target_acceptance_rate: float = 0.5
     min_stepsize_scale: float = 1e-2
     max_stepsize_scale: float = 1.0
-    proposal: MCMCSimpleProposalConfig = field(default_factory=MCMCSimpleProposalConfig)
+    proposal: Union[MCMCSimpleProposalConfig, LocalStepsizeProposalConfig, MCMCLangevinProposalConfig] = field(
+        default_factory=MCMCSimpleProposalConfig)
     p_spin_swap: float = 0.0
     p_spin_flip: float = 0.0
     stepsize: float = 0.02
 
     def validate(self) -> None:
-        if self.proposal.name != "normal":
-            raise NotImplementedError(f"proposal {self.proposal.name!r} is not on the native path")
+        if self.proposal.name not in ("normal", "local", "langevin"):
+            raise NotImplementedError(f"proposal {self.proposal.name!r} is not on the native path "
+                                      "(the device sampler has 'normal', 'local' and 'langevin')")
         if self.p_spin_swap > 0 or self.p_spin_flip > 0:
             raise NotImplementedError("spin swap / spin flip moves are not on the native path")
 

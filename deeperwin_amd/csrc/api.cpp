@@ -164,7 +164,9 @@ struct dw_model {
   DevBuf grad_arena;        // scratch of the parameter-gradient pass
   DevBuf kappa_w;           // per-walker conditioning of the last pass (dw_model_conditioning)
   long kappa_n = 0;         // walkers covered by kappa_w
-  DevBuf rprop, lpprop, cnt;
+  DevBuf rprop, lpprop, logq, cnt;
+  int prop_kind = DW_PROPOSAL_NORMAL;  // dw_model_set_proposal
+  float prop_rmin = 0.1f, prop_rmax = 1.0f, prop_scale = 1.0f;
   DevBuf sub_r, sub_e, sub_lp, sub_idx, own_flags;  // FP64 recompute of flagged walkers
   // measurement hooks
   bool timing = false;
@@ -183,8 +185,8 @@ struct dw_model {
     for (auto e : event_pool) (void)hipEventDestroy(e);
     wf.release();
     wd.release();
-    for (DevBuf* b : {&arena_f, &arena_d, &grad_arena, &kappa_w, &rprop, &lpprop, &cnt, &sub_r, &sub_e, &sub_lp,
-                      &sub_idx, &own_flags})
+    for (DevBuf* b : {&arena_f, &arena_d, &grad_arena, &kappa_w, &rprop, &lpprop, &logq, &cnt, &sub_r, &sub_e,
+                      &sub_lp, &sub_idx, &own_flags})
       b->release();
   }
 };
@@ -714,7 +716,8 @@ dw_status mcmc_run(dw_model* m, const McmcArgs& a, const FixedScratch* fx, hipSt
   const bool dbl = m->prec == GEMM_F64;
   const long B = a.n_walkers;
   const int cnt3 = (int)(B * m->n * 3);
-  float *rprop, *lpprop;
+  const bool scaled = m->prop_kind != DW_PROPOSAL_NORMAL;
+  float *rprop, *lpprop, *logq = nullptr;
   int32_t* cnt;
   PassBufs<float> pf;
   PassBufs<double> pd;
@@ -724,9 +727,10 @@ dw_status mcmc_run(dw_model* m, const McmcArgs& a, const FixedScratch* fx, hipSt
     Carver cv{base, pass_bytes(m, B, 1, dbl)};  // the pass's layout first, then the chain's own buffers
     rprop = cv.take<float>((size_t)cnt3);
     lpprop = cv.take<float>((size_t)B);
+    if (scaled) logq = cv.take<float>((size_t)B);
     cnt = cv.take<int32_t>((size_t)std::max(a.n_steps, 1));
     if (cv.off > scratch_bytes) {
-      const size_t per_walker = (pass_bytes(m, 64, 1, dbl) + 64 * (size_t)(m->n * 3 + 1) * 4) / 64;
+      const size_t per_walker = (pass_bytes(m, 64, 1, dbl) + 64 * (size_t)(m->n * 3 + 2) * 4) / 64;
       return fail(DW_ERR_INVALID, "dw_mcmc_steps_noalloc: " + std::to_string(B) + " walkers need " +
                                       std::to_string(cv.off) + " bytes of scratch, more than its " +
                                       std::to_string(scratch_bytes) + " (about " +
@@ -750,6 +754,10 @@ dw_status mcmc_run(dw_model* m, const McmcArgs& a, const FixedScratch* fx, hipSt
     rprop = m->rprop.as<float>();
     lpprop = m->lpprop.as<float>();
     cnt = m->cnt.as<int32_t>();
+    if (scaled) {
+      HIP_TRY(m->logq.ensure((size_t)B * 4));
+      logq = m->logq.as<float>();
+    }
     st = dbl ? pass_scratch(m, B, 1, &pd) : pass_scratch(m, B, 1, &pf);
     if (st) return st;
   }
@@ -765,11 +773,15 @@ dw_status mcmc_run(dw_model* m, const McmcArgs& a, const FixedScratch* fx, hipSt
   if (!st && a.n_steps > 0) {
     launch_fill_i32(cnt, 0, a.n_steps, s);
     for (int k = 0; k < a.n_steps && !st; ++k) {
-      launch_propose(a.r, a.normals + (size_t)k * cnt3, a.chain, rprop, cnt3, s);
+      if (scaled)
+        launch_propose_scaled(m->prop_kind, a.r, a.normals + (size_t)k * cnt3, a.chain, a.R, a.Z, m->I, m->prop_rmin,
+                              m->prop_rmax, m->prop_scale, rprop, logq, (int)B, m->n, s);
+      else
+        launch_propose(a.r, a.normals + (size_t)k * cnt3, a.chain, rprop, cnt3, s);
       st = value_pass(rprop, lpprop);
       if (st) break;
-      launch_accept(a.r, rprop, a.log_psi_sqr, lpprop, a.uniforms + (size_t)k * B, a.walker_age, a.max_age, cnt + k,
-                    a.out_flags, (int)B, m->n, s);
+      launch_accept(a.r, rprop, a.log_psi_sqr, lpprop, logq, a.uniforms + (size_t)k * B, a.walker_age, a.max_age,
+                    cnt + k, a.out_flags, (int)B, m->n, s);
       launch_chain_update(a.chain, cnt + k, (int)B, a.out_acc ? a.out_acc + k : nullptr, a.stepsize_update_interval,
                           a.target_acceptance, a.min_stepsize, a.max_stepsize, s);
     }
@@ -1111,6 +1123,37 @@ dw_status dw_threefry_mcmc_draws(const uint32_t* keys_in, uint32_t* keys_out, in
   if (keys_in == keys_out) return fail(DW_ERR_INVALID, "keys_out must not alias keys_in");
   if (n_walkers <= 0 || n_el <= 0 || n_steps <= 0) return fail(DW_ERR_INVALID, "bad sizes");
   launch_threefry_mcmc(keys_in, keys_out, n_walkers, n_el, n_steps, normals, uniforms, (hipStream_t)stream);
+  HIP_TRY(hipGetLastError());
+  return DW_OK;
+}
+
+dw_status dw_model_set_proposal(dw_model* m, int32_t kind, float r_min, float r_max, float langevin_scale) {
+  if (!m) return fail(DW_ERR_INVALID, "null model");
+  if (kind < DW_PROPOSAL_NORMAL || kind > DW_PROPOSAL_LANGEVIN) return fail(DW_ERR_INVALID, "unknown proposal kind");
+  if (kind != DW_PROPOSAL_NORMAL &&
+      !(r_min > 0.f && r_max >= r_min && std::isfinite(r_max) && std::isfinite(langevin_scale)))
+    return fail(DW_ERR_INVALID, "need 0 < r_min <= r_max (finite) and a finite langevin_scale");
+  m->prop_kind = kind;
+  m->prop_rmin = r_min;
+  m->prop_rmax = r_max;
+  m->prop_scale = langevin_scale;
+  return DW_OK;
+}
+
+dw_status dw_mcmc_propose(dw_model* m, const float* r, int32_t n_walkers, const float* R, const float* Z,
+                          const float* normals, const dw_chain_scalars* chain, float* out_r, float* out_log_q,
+                          dw_stream stream) {
+  if (!m || !r || !R || !Z || !normals || !chain || !out_r || !out_log_q) return fail(DW_ERR_INVALID, "null argument");
+  if (n_walkers <= 0) return fail(DW_ERR_INVALID, "n_walkers must be positive");
+  if (out_r == r) return fail(DW_ERR_INVALID, "out_r must not alias r");
+  const hipStream_t s = (hipStream_t)stream;
+  if (m->prop_kind == DW_PROPOSAL_NORMAL) {
+    launch_propose(r, normals, chain, out_r, n_walkers * m->n * 3, s);
+    launch_fill_i32((int32_t*)out_log_q, 0, n_walkers, s);  // +0.0f; a kernel, so the call can be captured
+  } else {
+    launch_propose_scaled(m->prop_kind, r, normals, chain, R, Z, m->I, m->prop_rmin, m->prop_rmax, m->prop_scale,
+                          out_r, out_log_q, n_walkers, m->n, s);
+  }
   HIP_TRY(hipGetLastError());
   return DW_OK;
 }

@@ -107,8 +107,15 @@ void launch_clip_energies(const float* e, int n, int tanh_clip, int median, int 
 void launch_gather_walkers(const float* r, const int32_t* idx, int count, int n, float* out, hipStream_t s);
 void launch_scatter_values(const float* v, const int32_t* idx, int count, float* out, hipStream_t s);
 void launch_propose(const float* r, const float* normals, const void* chain, float* r_prop, int count, hipStream_t s);
-void launch_accept(float* r, const float* r_prop, float* lp, const float* lp_prop, const float* uniforms,
-                   int32_t* age, int max_age, int32_t* acc_count, uint32_t* flags, int B, int n, hipStream_t s);
+// Local-stepsize (kind DW_PROPOSAL_LOCAL) and approximate-Langevin (DW_PROPOSAL_LANGEVIN) proposals
+// (mcmc.py:218-236, 255-284): r_prop [B, n, 3] and the per-walker log proposal ratio logq [B].  n <= 64.
+void launch_propose_scaled(int kind, const float* r, const float* normals, const void* chain, const float* R,
+                           const float* Z, int I, float r_min, float r_max, float lscale, float* r_prop, float* logq,
+                           int B, int n, hipStream_t s);
+// logq (may be null for a symmetric proposal) is added to the exponent: accept iff exp(lp' - lp + logq) > u.
+void launch_accept(float* r, const float* r_prop, float* lp, const float* lp_prop, const float* logq,
+                   const float* uniforms, int32_t* age, int max_age, int32_t* acc_count, uint32_t* flags, int B,
+                   int n, hipStream_t s);
 void launch_chain_update(void* chain, const int32_t* acc_count, int B, float* out_acc, int interval, float target,
                          float lo, float hi, hipStream_t s);
 

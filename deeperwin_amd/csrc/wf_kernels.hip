@@ -1765,10 +1765,11 @@ void launch_propose(const float* r, const float* normals, const void* chain, flo
                      r_prop, count);
 }
 
-// One 64-thread group per walker: lane 0 decides, all lanes copy the accepted coordinates.
-// accept = exp(lp' - lp) > u  or  old age >= max_age;  age <- 0 / age + 1;  lp and r selected.
+// accept = exp(lp' - lp + log q) > u  or  old age >= max_age;  age <- 0 / age + 1;  lp and r selected.
+// log q is the proposal's log density ratio (absent, i.e. 0, for the symmetric Gaussian proposal).
 __global__ __launch_bounds__(256) void k_accept(float* __restrict__ r, const float* __restrict__ r_prop,
                                                 float* __restrict__ lp, const float* __restrict__ lp_prop,
+                                                const float* __restrict__ logq,
                                                 const float* __restrict__ uniforms, int32_t* __restrict__ age,
                                                 int max_age, int32_t* __restrict__ acc_count,
                                                 uint32_t* __restrict__ flags, int B, int n) {
@@ -1780,7 +1781,7 @@ __global__ __launch_bounds__(256) void k_accept(float* __restrict__ r, const flo
     const long b = b0 + tid;
     bool acc = false;
     if (b < B) {
-      const float p_accept = expf(lp_prop[b] - lp[b]);
+      const float p_accept = expf(logq ? lp_prop[b] - lp[b] + logq[b] : lp_prop[b] - lp[b]);
       const bool by_ratio = p_accept > uniforms[b];
       const int old_age = age[b];
       const bool forced = old_age >= max_age;
@@ -1802,10 +1803,11 @@ __global__ __launch_bounds__(256) void k_accept(float* __restrict__ r, const flo
     if (acc_sh[w]) r[base + t] = r_prop[base + t];
   }
 }
-void launch_accept(float* r, const float* r_prop, float* lp, const float* lp_prop, const float* uniforms,
-                   int32_t* age, int max_age, int32_t* acc_count, uint32_t* flags, int B, int n, hipStream_t s) {
-  hipLaunchKernelGGL(k_accept, dim3((unsigned)((B + 63) / 64)), dim3(256), 0, s, r, r_prop, lp, lp_prop, uniforms,
-                     age, max_age, acc_count, flags, B, n);
+void launch_accept(float* r, const float* r_prop, float* lp, const float* lp_prop, const float* logq,
+                   const float* uniforms, int32_t* age, int max_age, int32_t* acc_count, uint32_t* flags, int B,
+                   int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_accept, dim3((unsigned)((B + 63) / 64)), dim3(256), 0, s, r, r_prop, lp, lp_prop, logq,
+                     uniforms, age, max_age, acc_count, flags, B, n);
 }
 
 // acc = mean(accept); step_nr += 1; acc_rate = 0.9 acc_rate + 0.1 acc; every `interval` steps the

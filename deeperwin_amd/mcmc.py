@@ -2,9 +2,10 @@
 
 ``MetropolisHastingsMonteCarlo(mcmc_config).run_mcmc_steps(func, state, params, n_up, n_dn,
 fixed_params, n_steps)`` keeps the reference signature and returns a new ``MCMCState``.  The whole chain
-of ``n_steps`` Gaussian-proposal steps runs on the device without host synchronisation: proposals,
-log|psi|^2 of the proposals, accept/reject, walker ages, the running acceptance rate and the stepsize
-schedule (stepsize, acc_rate and step_nr live in a 16-byte device record).
+of ``n_steps`` steps runs on the device without host synchronisation: proposals (the config's 'normal',
+'local' or 'langevin' rule, mcmc.py:175-180, 218-236, 255-284), log|psi|^2 of the proposals,
+accept/reject, walker ages, the running acceptance rate and the stepsize schedule (stepsize, acc_rate and
+step_nr live in a 16-byte device record).
 
 Random draws: the reference derives proposals and accept thresholds from per-walker threefry keys
 (mcmc.py:178, 360-361).  When ``state.rng_state`` holds such keys (a CUDA int32 tensor [B, 2] of uint32
@@ -149,6 +150,9 @@ class MetropolisHastingsMonteCarlo:
             normals = uniforms = None
         acc = torch.empty(max(n_steps, 1), device="cuda", dtype=torch.float32)
         c = self.config
+        p = c.proposal
+        model.set_proposal(p.name, getattr(p, "r_min", 0.1), getattr(p, "r_max", 1.0),
+                           getattr(p, "langevin_scale", 1.0))
         model.mcmc_steps(new.r, _as_device(state.R), _as_device(state.Z), new.log_psi_sqr, new.walker_age, normals,
                          uniforms, new.chain, c.max_age, c.stepsize_update_interval, c.target_acceptance_rate,
                          c.min_stepsize_scale, c.max_stepsize_scale, out_acc=acc)

@@ -94,9 +94,31 @@ typedef struct {
   int32_t _pad;
 } dw_chain_scalars;
 
-/* n_steps Metropolis-Hastings steps with Gaussian proposals and SUPPLIED draws:
- * normals [n_steps, B, n_el, 3], uniforms [n_steps, B].  r, log_psi_sqr and walker_age are updated in
- * place; chain (device dw_chain_scalars) carries stepsize / acc_rate / step_nr; out_acc [n_steps]
+/* Proposal rules of the sampler (the `proposal` config of MCMCConfig, configuration.py:952-975, 1017-1021). */
+#define DW_PROPOSAL_NORMAL 0   /* r' = r + stepsize xi                                  (mcmc.py:175-180) */
+#define DW_PROPOSAL_LOCAL 1    /* per-electron stepsize from the closest nucleus        (mcmc.py:218-236) */
+#define DW_PROPOSAL_LANGEVIN 2 /* local stepsize plus a drift towards the nuclei        (mcmc.py:255-284) */
+
+/* Select the proposal of dw_mcmc_steps / dw_mcmc_steps_noalloc / dw_mcmc_propose for this model (default
+ * DW_PROPOSAL_NORMAL).  For LOCAL and LANGEVIN each electron moves with s = stepsize * clip(d, r_min, r_max),
+ * d its distance to the closest nucleus; LANGEVIN adds g s^2 with g = -langevin_scale sum_I Z_I (r - R_I) /
+ * |r - R_I|.  The accept test becomes exp(lp' - lp + log q) > u with log q the log ratio of the reverse to
+ * the forward proposal density.  The reference's defaults: local r_min 0.1, r_max 1; langevin r_min 0.2,
+ * r_max 2, langevin_scale 1 (configuration.py:956-975).  Needs 0 < r_min <= r_max.  Replaces the choice of
+ * self.propose in MetropolisHastingsMonteCarlo.__init__ (mcmc.py:330-342) for these three names. */
+dw_status dw_model_set_proposal(dw_model* m, int32_t kind, float r_min, float r_max, float langevin_scale);
+
+/* One proposal of the selected rule without evaluating the wavefunction: out_r [B, n_el, 3] and
+ * out_log_q [B] (0 for DW_PROPOSAL_NORMAL) from r [B, n_el, 3], normals [B, n_el, 3] and the chain's
+ * stepsize.  Replaces the propose functions `_propose_normal`, `_propose_step_local_stepsize` and
+ * `_propose_step_local_approximated_langevin` (mcmc.py:175-180, 218-236, 255-284) with supplied draws. */
+dw_status dw_mcmc_propose(dw_model* m, const float* r, int32_t n_walkers, const float* R, const float* Z,
+                          const float* normals, const dw_chain_scalars* chain, float* out_r, float* out_log_q,
+                          dw_stream stream);
+
+/* n_steps Metropolis-Hastings steps with the model's proposal rule (dw_model_set_proposal) and SUPPLIED
+ * draws: normals [n_steps, B, n_el, 3], uniforms [n_steps, B].  r, log_psi_sqr and walker_age are updated
+ * in place; chain (device dw_chain_scalars) carries stepsize / acc_rate / step_nr; out_acc [n_steps]
  * receives each step's acceptance fraction (may be NULL).  log_psi_sqr is re-evaluated on entry.
  * Replaces MetropolisHastingsMonteCarlo.run_mcmc_steps (mcmc.py:325-327, 345-406). */
 dw_status dw_mcmc_steps(dw_model* m, float* r, int32_t n_walkers, const float* R, const float* Z,

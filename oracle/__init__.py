@@ -76,6 +76,20 @@ PINNING = {
     "mcmc.adjust_stepsize": [
         "test_oracle_mcmc.py::test_stepsize_clip_and_increase",
     ],
+    "mcmc.propose": [
+        "test_oracle_proposals.py::test_log_q_is_reverse_over_forward_density (scipy.stats.norm densities)",
+        "test_oracle_proposals.py::test_stationary_distribution_hydrogen_1s (local and Langevin chains; fails "
+        "without log q)",
+        "test_oracle_proposals.py::test_constant_scale_without_drift_is_symmetric",
+    ],
+    "mcmc.log_q_ratio": [
+        "test_oracle_proposals.py::test_log_q_is_reverse_over_forward_density (through propose)",
+        "test_oracle_proposals.py::test_stationary_distribution_hydrogen_1s",
+    ],
+    "mcmc.scale_and_drift": [
+        "test_oracle_proposals.py::test_drift_is_minus_gradient_of_nuclear_distance_sum (central differences)",
+        "test_oracle_proposals.py::test_planted_mistakes_fail_the_density_check",
+    ],
     "hamiltonian.kinetic_energy_vmapped": [
         "test_oracle_physics.py::test_vmapped_kinetic_energy_equals_walker_loop (against the pinned walker loop)",
     ],

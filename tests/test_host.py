@@ -3,7 +3,8 @@ truncation, electron-to-ion mapping, seeded geometries and walker initialisation
 import numpy as np
 import pytest
 
-from deeperwin_amd.config import MCMCConfig, ModelConfigFermiNet, PhysicalConfig
+from deeperwin_amd.config import (LocalStepsizeProposalConfig, MCMCConfig, MCMCLangevinProposalConfig,
+                                 ModelConfigFermiNet, PhysicalConfig)
 from deeperwin_amd.params import ENV, ModelShape, find_leaf, infer_shape, init_params, truncate_params
 from deeperwin_amd.physics import (batch_geometries, generate_el_ion_mapping, geometry_c4h2, init_walkers_gaussian,
                                    random_molecule)
@@ -127,6 +128,8 @@ def test_config_validation():
     with pytest.raises(NotImplementedError):
         c.validate()
     m = MCMCConfig()
-    m.proposal.name = "langevin"
+    m.proposal.name = "cauchy"
     with pytest.raises(NotImplementedError):
         m.validate()
+    for prop in (LocalStepsizeProposalConfig(), MCMCLangevinProposalConfig()):
+        MCMCConfig(proposal=prop).validate()
