@@ -30,7 +30,11 @@ _vp, _i32, _f32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_float
 class ModelDesc(ctypes.Structure):
     _fields_ = [("n_el", ctypes.c_int32), ("n_up", ctypes.c_int32), ("n_ion", ctypes.c_int32),
                 ("n_det", ctypes.c_int32), ("width1", ctypes.c_int32), ("width2", ctypes.c_int32),
-                ("n_iter", ctypes.c_int32), ("precision", ctypes.c_int32), ("fp64_cond_threshold", ctypes.c_float)]
+                ("n_iter", ctypes.c_int32), ("precision", ctypes.c_int32), ("fp64_cond_threshold", ctypes.c_float),
+                ("embedding", ctypes.c_int32), ("z_min", ctypes.c_int32), ("n_ion_types", ctypes.c_int32)]
+
+
+EMBEDDINGS = {"ferminet": 0, "dpe4": 1}
 
 
 class Tuning(ctypes.Structure):
@@ -128,7 +132,8 @@ class NativeModel:
         self.shape = shape
         self.precision = precision
         desc = ModelDesc(shape.n_el, shape.n_up, shape.n_ion, shape.n_det, shape.width1, shape.width2,
-                         shape.n_iter, PRECISIONS[precision], float(fp64_cond_threshold))
+                         shape.n_iter, PRECISIONS[precision], float(fp64_cond_threshold),
+                         EMBEDDINGS[shape.embedding], int(shape.z_min), int(shape.n_ion_types))
         h = ctypes.c_void_p()
         _check(self.lib.dw_model_create(ctypes.byref(desc), ctypes.byref(h)))
         self.handle = h

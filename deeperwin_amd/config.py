@@ -91,6 +91,78 @@ class ModelConfigFermiNet:
 
 
 @dataclass
+class EmbeddingConfigDeepErwin4(EmbeddingConfigFermiNet):
+    """The dpe4 embedding flags (configuration.py:422-441)."""
+    name: str = "dpe4"
+    use_el_ion_stream: bool = True
+    use_h_two_same_diff: bool = True
+    emb_dim: int = 32
+    use_w_mapping: bool = True
+    use_schnet_features: bool = True
+    sum_schnet_features: bool = False
+    use_average_h_two: bool = False
+    use_linear_out: bool = False
+    use_schnet_bias_feat: bool = True
+    schnet_aggregation: str = "sum"
+    use_deep_schnet_feat: bool = False
+    h_one_correlation: int = 0
+
+
+@dataclass
+class InputFeatureConfigDPE4:
+    """dpe4 input features (configuration.py:643-674): el-ion [dist, diff] concatenated into h_one, the
+    el-el distance without differences, the ion embedding table ('lookup', 32 features)."""
+    name: str = "dpe4"
+    use_rbf_features: bool = False
+    use_distance_features: bool = True
+    use_el_ion_differences: bool = True
+    use_el_el_differences: bool = False
+    log_scale_distances: bool = False
+    concatenate_el_ion_features: bool = True
+    ion_embed_type: str = "lookup"
+    n_ion_features: int = 32
+    use_el_spin: bool = False
+    coordinates: str = "cartesian"
+
+
+@dataclass
+class ModelConfigDeepErwin4(ModelConfigFermiNet):
+    """The reference's default model (ModelConfigDeepErwin4, configuration.py:875-891).  Z_min / Z_max fix
+    the ion embedding table (by default the smallest and largest charge of the molecule,
+    model/wavefunction.py:310-326)."""
+    name: str = "dpe4"
+    embedding: EmbeddingConfigDeepErwin4 = field(default_factory=EmbeddingConfigDeepErwin4)
+    features: InputFeatureConfigDPE4 = field(default_factory=InputFeatureConfigDPE4)
+    Z_min: Optional[int] = None
+    Z_max: Optional[int] = None
+
+    def validate(self) -> None:
+        e, f, o, m = self.embedding, self.features, self.orbitals, self.mlp
+        if e.name != "dpe4" or f.name != "dpe4":
+            raise NotImplementedError("ModelConfigDeepErwin4 runs the dpe4 embedding and features")
+        if m.activation != "tanh" or m.use_layer_norm or m.use_residual:
+            raise NotImplementedError("native MLP path is tanh, no layer norm, no global residual")
+        flags = (e.use_el_ion_stream, e.use_h_two_same_diff, e.use_w_mapping, e.use_schnet_features,
+                 not e.sum_schnet_features, not e.use_average_h_two, e.use_average_h_one, e.use_h_one,
+                 not e.use_linear_out, e.use_schnet_bias_feat, e.schnet_aggregation == "sum",
+                 not e.use_deep_schnet_feat, e.h_one_correlation == 0, e.emb_dim == 32, e.n_hidden_two_el == 32)
+        if not all(flags):
+            raise NotImplementedError("only the default dpe4 embedding flags are implemented")
+        if (f.use_rbf_features or not f.use_distance_features or not f.use_el_ion_differences or
+                f.use_el_el_differences or f.log_scale_distances or not f.concatenate_el_ion_features or
+                f.ion_embed_type != "lookup" or f.n_ion_features != 32 or f.use_el_spin or f.coordinates != "cartesian"):
+            raise NotImplementedError("only the default dpe4 input features are implemented")
+        if o.determinant_schema != "full_det":
+            raise NotImplementedError("only determinant_schema='full_det' is implemented")
+        if o.envelope_orbitals.envelope_type != "isotropic_exp" or o.envelope_orbitals.n_hidden:
+            raise NotImplementedError("only isotropic_exp envelopes without hidden backflow layers")
+        if o.envelope_orbitals.use_bias:
+            raise NotImplementedError("backflow output bias is off")
+        if self.jastrow or self.use_el_el_cusp_correction or self.complex_wf:
+            raise NotImplementedError("jastrow / cusp / complex wavefunctions are off in the dpe4 model")
+
+
+@dataclass
 class MCMCSimpleProposalConfig:
     name: str = "normal"
 

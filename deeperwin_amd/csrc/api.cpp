@@ -112,11 +112,15 @@ struct Weights {
   std::vector<DevBuf> Wmain, Wmu, bias, Wmain_hi, Wmain_lo, Wmu_hi, Wmu_lo;
   std::vector<DevBuf> WmainT, WmuT;  // transposed copies for the gradient pass ([pad128(K)][W1])
   DevBuf W2e, b2e, Wo[2], Wo_hi[2], Wo_lo[2], WoT[2], kappa, wenv;
+  // dpe4: pair-stream sets, el-ion stream, ion maps, ion table; h_map GEMM operands per iteration
+  DevBuf Wp, bp, Wei, bei, Whim, bhim, table;
+  std::vector<DevBuf> WhmT, WhmT_hi, WhmT_lo, bhm;
   void release() {
-    for (auto* v : {&Wmain, &Wmu, &bias, &Wmain_hi, &Wmain_lo, &Wmu_hi, &Wmu_lo, &WmainT, &WmuT})
+    for (auto* v : {&Wmain, &Wmu, &bias, &Wmain_hi, &Wmain_lo, &Wmu_hi, &Wmu_lo, &WmainT, &WmuT, &WhmT, &WhmT_hi,
+                    &WhmT_lo, &bhm})
       for (auto& b : *v) b.release();
     for (DevBuf* b : {&W2e, &b2e, &Wo[0], &Wo[1], &Wo_hi[0], &Wo_hi[1], &Wo_lo[0], &Wo_lo[1], &WoT[0], &WoT[1], &kappa,
-                      &wenv})
+                      &wenv, &Wp, &bp, &Wei, &bei, &Whim, &bhim, &table})
       b->release();
     ready = false;
   }
@@ -129,6 +133,7 @@ template <typename T>
 struct PassBufs {
   T* X[4] = {nullptr, nullptr, nullptr, nullptr};
   T *MU = nullptr, *V = nullptr, *Z = nullptr, *H = nullptr, *O = nullptr, *A = nullptr;
+  T *WM = nullptr, *HM = nullptr, *HIM = nullptr;  // dpe4: mapped pairs, h_map rows, ion maps
   double *ld = nullptr, *sd = nullptr, *g = nullptr, *lap = nullptr;
   float* dcond = nullptr;
 };
@@ -151,6 +156,9 @@ struct dw_model {
   dw_model_desc d{};
   int n = 0, u = 0, dn = 0, I = 0, D = 0, W1 = 0, W2 = 0, L = 0, NO = 0, NOp = 0, NOk = 0;
   std::vector<int> Kh, Ke, Kx, Km;
+  std::vector<int> Kt0, Kt;  // columns [Kt0, Kt0 + Kt) of X_l: the 2-el block (pair means, or e_ee | e_ei)
+  bool dpe4 = false;         // DW_EMBEDDING_DPE4
+  int z_min = 1, types = 0;  // dpe4 ion table
   std::map<std::string, Leaf> leaves;  // key "<module>:<leaf>"
   std::vector<std::pair<std::string, std::vector<int64_t>>> expected;
   bool dirty = true;
@@ -197,15 +205,27 @@ std::string leaf_key(const std::string& module, const std::string& leaf) { retur
 
 // Identify a leaf from its (normalised) module path.  Returns "" if unknown.
 std::string classify(const dw_model* m, const std::string& path, const std::string& leaf) {
+  const bool wb = leaf == "w" || leaf == "b";
   for (int l = 0; l < m->L; ++l) {
-    char buf[64];
+    char buf[96];
     std::snprintf(buf, sizeof buf, "h_el_%d/linear_0", l);
-    if (ends_with(path, buf) && (leaf == "w" || leaf == "b")) return leaf_key(buf, leaf);
-    if (l < m->L - 1) {
+    if (ends_with(path, buf) && wb) return leaf_key(buf, leaf);
+    if (m->dpe4) {
+      for (const char* mod : {"w_same", "w_diff", "h_map", "h_ion_map"}) {
+        std::snprintf(buf, sizeof buf, "symm_features_%d/convolutional_features/%s/linear_0", l, mod);
+        if (ends_with(path, buf) && wb) return leaf_key(buf, leaf);
+      }
+      if (l < m->L - 1)
+        for (const char* mod : {"h_same", "h_diff", "h_el_ion"}) {
+          std::snprintf(buf, sizeof buf, "%s_%d/linear_0", mod, l);
+          if (ends_with(path, buf) && wb) return leaf_key(buf, leaf);
+        }
+    } else if (l < m->L - 1) {
       std::snprintf(buf, sizeof buf, "h_el_el_%d/linear_0", l);
-      if (ends_with(path, buf) && (leaf == "w" || leaf == "b")) return leaf_key(buf, leaf);
+      if (ends_with(path, buf) && wb) return leaf_key(buf, leaf);
     }
   }
+  if (m->dpe4 && ends_with(path, "input/h_ion") && leaf == "embeddings") return leaf_key("input/h_ion", leaf);
   if (ends_with(path, "bf_up/linear_0") && leaf == "w") return leaf_key("bf_up/linear_0", "w");
   if (ends_with(path, "bf_dn/linear_0") && leaf == "w") return leaf_key("bf_dn/linear_0", "w");
   if (ends_with(path, "envelope_orbitals") &&
@@ -238,6 +258,74 @@ dw_status upload_split(DevBuf& hi, DevBuf& lo, const std::vector<float>& w) {
   return upload(lo, l);
 }
 
+// dpe4 weights: the pair-stream sets Wp [2L-1][2][32][32] (maps w_same/w_diff of every iteration, then the
+// stream layers h_same/h_diff), the el-ion stream Wei [L-1][32][32], the ion maps Whim [L][32][32] (output
+// columns beyond the iteration's width zero), the ion table, and per iteration the h_map GEMM operand
+// WhmT[l] [128][pad32(Kh)] (row = output feature) with its bias.  Fan-in rows beyond a layer's width are 0.
+dw_status prepare_dpe4(dw_model* m, Weights& W, bool dbl, bool split) {
+  const int L = m->L, E = 32, NS = 2 * L - 1;
+  auto up = [&](DevBuf& b, const std::vector<float>& v) { return dbl ? upload_as<double>(b, v) : upload(b, v); };
+  auto leafp = [&](const std::string& mod, const char* leaf) -> const Leaf& { return m->leaves[leaf_key(mod, leaf)]; };
+  std::vector<float> wp((size_t)NS * 2 * E * E, 0.f), bpv((size_t)NS * 2 * E, 0.f);
+  for (int q = 0; q < NS; ++q) {
+    const int l = q < L ? q : q - L;
+    const int kin = l == 0 ? 1 : E;
+    for (int sp = 0; sp < 2; ++sp) {
+      char buf[96];
+      if (q < L)
+        std::snprintf(buf, sizeof buf, "symm_features_%d/convolutional_features/%s/linear_0", l,
+                      sp == 0 ? "w_same" : "w_diff");
+      else
+        std::snprintf(buf, sizeof buf, "%s_%d/linear_0", sp == 0 ? "h_same" : "h_diff", l);
+      const Leaf& w = leafp(buf, "w");
+      const Leaf& b = leafp(buf, "b");
+      for (int k = 0; k < kin; ++k)
+        for (int f = 0; f < E; ++f) wp[(((size_t)q * 2 + sp) * E + k) * E + f] = w.data[(size_t)k * E + f];
+      for (int f = 0; f < E; ++f) bpv[((size_t)q * 2 + sp) * E + f] = b.data[f];
+    }
+  }
+  const int NE = L - 1 > 0 ? L - 1 : 1;
+  std::vector<float> wei((size_t)NE * E * E, 0.f), beiv((size_t)NE * E, 0.f);
+  for (int l = 0; l < L - 1; ++l) {
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "h_el_ion_%d/linear_0", l);
+    const int kin = l == 0 ? 4 : E;
+    const Leaf& w = leafp(buf, "w");
+    const Leaf& b = leafp(buf, "b");
+    for (int k = 0; k < kin; ++k)
+      for (int f = 0; f < E; ++f) wei[((size_t)l * E + k) * E + f] = w.data[(size_t)k * E + f];
+    for (int f = 0; f < E; ++f) beiv[(size_t)l * E + f] = b.data[f];
+  }
+  std::vector<float> whim((size_t)L * E * E, 0.f), bhimv((size_t)L * E, 0.f);
+  for (int l = 0; l < L; ++l) {
+    char buf[96];
+    std::snprintf(buf, sizeof buf, "symm_features_%d/convolutional_features/h_ion_map/linear_0", l);
+    const int g = l == 0 ? 4 : E;
+    const Leaf& w = leafp(buf, "w");
+    const Leaf& b = leafp(buf, "b");
+    for (int k = 0; k < E; ++k)
+      for (int f = 0; f < g; ++f) whim[((size_t)l * E + k) * E + f] = w.data[(size_t)k * g + f];
+    for (int f = 0; f < g; ++f) bhimv[(size_t)l * E + f] = b.data[f];
+  }
+  dw_status st;
+  if ((st = up(W.Wp, wp)) || (st = up(W.bp, bpv)) || (st = up(W.Wei, wei)) || (st = up(W.bei, beiv)) ||
+      (st = up(W.Whim, whim)) || (st = up(W.bhim, bhimv)) || (st = up(W.table, leafp("input/h_ion", "embeddings").data)))
+    return st;
+  for (auto* v : {&W.WhmT, &W.WhmT_hi, &W.WhmT_lo, &W.bhm}) v->resize(L);
+  for (int l = 0; l < L; ++l) {
+    char buf[96];
+    std::snprintf(buf, sizeof buf, "symm_features_%d/convolutional_features/h_map/linear_0", l);
+    const Leaf& w = leafp(buf, "w");
+    const int Kh = m->Kh[l], Khp = m->Kt0[l];
+    std::vector<float> wt((size_t)128 * Khp, 0.f);
+    for (int f = 0; f < E; ++f)
+      for (int k = 0; k < Kh; ++k) wt[(size_t)f * Khp + k] = w.data[(size_t)k * E + f];
+    if ((st = up(W.WhmT[l], wt)) || (st = up(W.bhm[l], leafp(buf, "b").data))) return st;
+    if (split && (st = upload_split(W.WhmT_hi[l], W.WhmT_lo[l], wt))) return st;
+  }
+  return DW_OK;
+}
+
 // Build the device layouts of all weights from the host leaves, in float or double.
 dw_status prepare_weights(dw_model* m, Weights& W, bool dbl) {
   for (auto& e : m->expected)
@@ -252,12 +340,14 @@ dw_status prepare_weights(dw_model* m, Weights& W, bool dbl) {
     std::snprintf(buf, sizeof buf, "h_el_%d/linear_0", l);
     const Leaf& w = m->leaves[leaf_key(buf, "w")];
     const Leaf& b = m->leaves[leaf_key(buf, "b")];
-    const int Kh = m->Kh[l], Ke = m->Ke[l], Kx = m->Kx[l], Km = m->Km[l];
-    // reference row blocks of W: [h (Kh) | mean_up h (Kh) | mean_dn h (Kh) | pair up (Ke) | pair dn (Ke)]
+    const int Kh = m->Kh[l], Kx = m->Kx[l], Km = m->Km[l], Kt0 = m->Kt0[l], Kt = m->Kt[l];
+    // reference row blocks of W: [h (Kh) | mean_up h (Kh) | mean_dn h (Kh) | 2-el block (Kt)], the 2-el block
+    // being [pair up | pair dn] (FermiNet) or [e_ee | e_ei] (dpe4); X_l holds h in columns [0, Kh) and the
+    // 2-el block in [Kt0, Kt0 + Kt)
     std::vector<float> wm((size_t)W1 * Kx, 0.f), wu((size_t)W1 * Km, 0.f);
     for (int o = 0; o < W1; ++o) {
       for (int k = 0; k < Kh; ++k) wm[(size_t)o * Kx + k] = w.data[(size_t)k * W1 + o];
-      for (int k = 0; k < 2 * Ke; ++k) wm[(size_t)o * Kx + Kh + k] = w.data[(size_t)(3 * Kh + k) * W1 + o];
+      for (int k = 0; k < Kt; ++k) wm[(size_t)o * Kx + Kt0 + k] = w.data[(size_t)(3 * Kh + k) * W1 + o];
       for (int k = 0; k < 2 * Kh; ++k) wu[(size_t)o * Km + k] = w.data[(size_t)(Kh + k) * W1 + o];
     }
     // transposed copies [pad128(K)][W1] for the gradient pass (dX = dz W^T as C = A Bt^T)
@@ -277,7 +367,10 @@ dw_status prepare_weights(dw_model* m, Weights& W, bool dbl) {
       if ((st = upload_split(W.Wmu_hi[l], W.Wmu_lo[l], wu))) return st;
     }
   }
-  {
+  if (m->dpe4) {
+    dw_status st = prepare_dpe4(m, W, dbl, split);
+    if (st) return st;
+  } else {
     const int NL2 = m->L - 1 > 0 ? m->L - 1 : 1;
     std::vector<float> w2((size_t)NL2 * 32 * 32, 0.f), b2((size_t)NL2 * 32, 0.f);
     for (int l = 0; l < m->L - 1; ++l) {
@@ -379,6 +472,11 @@ size_t layout_pass(const dw_model* m, long B, int C, char* base, PassBufs<T>* ou
   b.H = cv.take<T>((size_t)rows * m->W1);
   b.O = cv.take<T>((size_t)rows * m->NO);
   if (!det_reads_orbitals(m->n, C)) b.A = cv.take<T>((size_t)B * m->D * C * m->n * m->n);  // envelope x orbitals
+  if (m->dpe4) {
+    b.WM = cv.take<T>((size_t)m->L * B * m->n * m->n * (C > 1 ? 5 : 1) * 32);
+    b.HM = cv.take<T>((size_t)rows * 32);
+    b.HIM = cv.take<T>((size_t)m->L * m->I * 32);
+  }
   b.ld = cv.take<double>((size_t)B * m->D);
   b.sd = cv.take<double>((size_t)B * m->D);
   b.dcond = cv.take<float>((size_t)B * m->D);
@@ -465,10 +563,24 @@ dw_status run_pass(dw_model* m, const PassBufs<T>& ws, const float* r, long B, c
   const long rows = B * m->n * Cp;
   Timed pass_timer(m, C > 1 ? DW_TIMER_LOCAL_ENERGY : DW_TIMER_FORWARD, s);
   T* const* X = ws.X;
-  launch_features0<T>(r, R, X[0], m->Kx[0], sh, s);
-  launch_twoel<T>(r, X, m->Kx.data(), m->Kh.data(), W.W2e.as<T>(), W.b2e.as<T>(), sh, s);
+  launch_features0<T>(r, R, X[0], m->Kx[0], m->Kt0[0], m->Kt[0], sh, s);
+  if (m->dpe4) {
+    launch_dpe4_pairs<T>(r, W.Wp.as<T>(), W.bp.as<T>(), ws.WM, sh, s);
+    launch_dpe4_elion<T>(r, R, Z, W.Wei.as<T>(), W.bei.as<T>(), W.Whim.as<T>(), W.bhim.as<T>(), W.table.as<T>(),
+                         m->types, m->z_min, ws.HIM, X, m->Kx.data(), m->Kt0.data(), sh, s);
+  } else {
+    launch_twoel<T>(r, X, m->Kx.data(), m->Kh.data(), W.W2e.as<T>(), W.b2e.as<T>(), sh, s);
+  }
   const long vstride = B * Cp * (long)m->W1;
+  const long wm_layer = B * (long)m->n * m->n * (C > 1 ? 5 : 1) * 32;
   for (int l = 0; l < m->L; ++l) {
+    if (m->dpe4) {  // e_ee of this iteration: hm = tanh rule of (X_l[:, h] W_map), then the convolution
+      GemmEpiT<T> eh;
+      gemm_t(m, X[l], m->Kx[l], W.WhmT[l].as<T>(), W.WhmT_hi[l], W.WhmT_lo[l], m->Kt0[l], ws.HM, 32, rows, 32,
+             m->Kt0[l], eh, s);
+      launch_dpe4_tanh<T>(ws.HM, W.bhm[l].as<T>(), sh, s);
+      launch_dpe4_conv<T>(ws.WM + l * wm_layer, ws.HM, X[l], m->Kx[l], m->Kt0[l], sh, s);
+    }
     launch_spin_mean<T>(X[l], m->Kx[l], m->Kh[l], ws.MU, m->Km[l], sh, s);
     // value pass: split K of the per-walker spin-mean GEMM so that it fills the chip (its M is only B)
     int vs = 1;
@@ -838,6 +950,10 @@ dw_status dw_model_create(const dw_model_desc* desc, dw_model** out) {
   if (d.width1 < 128 || d.width1 % 128 != 0) return fail(DW_ERR_INVALID, "width1 must be a multiple of 128");
   if (d.n_iter < 2 || d.n_iter > 4) return fail(DW_ERR_INVALID, "n_iter must be in [2, 4]");
   if (d.precision < DW_FP32 || d.precision > DW_FP64) return fail(DW_ERR_INVALID, "unknown precision");
+  if (d.embedding != DW_EMBEDDING_FERMINET && d.embedding != DW_EMBEDDING_DPE4)
+    return fail(DW_ERR_INVALID, "unknown embedding");
+  if (d.embedding == DW_EMBEDDING_DPE4 && (d.n_ion_types < 1 || d.n_ion_types > 128))
+    return fail(DW_ERR_INVALID, "dpe4 needs 1 <= n_ion_types <= 128");
   dw_model* m = new dw_model();
   m->d = d;
   m->n = d.n_el;
@@ -852,25 +968,52 @@ dw_status dw_model_create(const dw_model_desc* desc, dw_model** out) {
   m->NOp = pad_to(m->NO, 128);
   m->NOk = pad_to(m->NO, 32);
   m->prec = (GemmPrec)d.precision;
+  m->dpe4 = d.embedding == DW_EMBEDDING_DPE4;
+  m->z_min = d.z_min;
+  m->types = d.n_ion_types;
   for (int l = 0; l < m->L; ++l) {
     const int Kh = (l == 0) ? 4 * m->I : m->W1;
     const int Ke = (l == 0) ? 4 : m->W2;
     m->Kh.push_back(Kh);
     m->Ke.push_back(Ke);
-    m->Kx.push_back(pad_to(Kh + 2 * Ke, 32));
+    // dpe4: X_l = [h | zeros to a multiple of 32 | e_ee (32) | e_ei (4 at l = 0, else 32)], so that the
+    // h_map GEMM reads the h columns with K a multiple of 32
+    const int t0 = m->dpe4 ? pad_to(Kh, 32) : Kh;
+    const int tw = m->dpe4 ? 32 + (l == 0 ? 4 : 32) : 2 * Ke;
+    m->Kt0.push_back(t0);
+    m->Kt.push_back(tw);
+    m->Kx.push_back(pad_to(t0 + tw, 32));
     m->Km.push_back(pad_to(2 * Kh, 32));
   }
   for (int l = 0; l < m->L; ++l) {
-    char buf[64];
+    char buf[96];
     std::snprintf(buf, sizeof buf, "h_el_%d/linear_0", l);
-    m->expected.push_back({leaf_key(buf, "w"), {3 * m->Kh[l] + 2 * m->Ke[l], m->W1}});
+    m->expected.push_back({leaf_key(buf, "w"), {3 * m->Kh[l] + m->Kt[l], m->W1}});
     m->expected.push_back({leaf_key(buf, "b"), {m->W1}});
-    if (l < m->L - 1) {
+    if (m->dpe4) {
+      const int fin = l == 0 ? 1 : 32, g = l == 0 ? 4 : 32;
+      const struct { const char* mod; int64_t kin, kout; } conv[4] = {
+          {"w_same", fin, 32}, {"w_diff", fin, 32}, {"h_map", m->Kh[l], 32}, {"h_ion_map", 32, g}};
+      for (auto& c : conv) {
+        std::snprintf(buf, sizeof buf, "symm_features_%d/convolutional_features/%s/linear_0", l, c.mod);
+        m->expected.push_back({leaf_key(buf, "w"), {c.kin, c.kout}});
+        m->expected.push_back({leaf_key(buf, "b"), {c.kout}});
+      }
+      if (l < m->L - 1) {
+        const struct { const char* mod; int64_t kin; } strm[3] = {{"h_same", fin}, {"h_diff", fin}, {"h_el_ion", g}};
+        for (auto& c : strm) {
+          std::snprintf(buf, sizeof buf, "%s_%d/linear_0", c.mod, l);
+          m->expected.push_back({leaf_key(buf, "w"), {c.kin, 32}});
+          m->expected.push_back({leaf_key(buf, "b"), {32}});
+        }
+      }
+    } else if (l < m->L - 1) {
       std::snprintf(buf, sizeof buf, "h_el_el_%d/linear_0", l);
       m->expected.push_back({leaf_key(buf, "w"), {l == 0 ? 4 : m->W2, m->W2}});
       m->expected.push_back({leaf_key(buf, "b"), {m->W2}});
     }
   }
+  if (m->dpe4) m->expected.push_back({leaf_key("input/h_ion", "embeddings"), {m->types, 32}});
   m->expected.push_back({leaf_key("bf_up/linear_0", "w"), {m->W1, m->NO}});
   m->expected.push_back({leaf_key("bf_dn/linear_0", "w"), {m->W1, m->NO}});
   for (const char* k : {"alpha_up", "alpha_dn", "weights_up", "weights_dn"})
@@ -1093,6 +1236,7 @@ dw_status dw_param_grad(dw_model* m, const float* r, int32_t n_walkers, const fl
   if (!m || !r || !R || !Z || !walker_weights || !out_grad) return fail(DW_ERR_INVALID, "null argument");
   if (n_walkers <= 0) return fail(DW_ERR_INVALID, "n_walkers must be positive");
   if (m->n > 52) return fail(DW_ERR_INVALID, "dw_param_grad supports up to 52 electrons (the FP64 inverse in LDS)");
+  if (m->dpe4) return fail(DW_ERR_INVALID, "dw_param_grad covers the FermiNet embedding only, not dpe4");
   hipStream_t s = (hipStream_t)stream;
   dw_status st = prepare(m, false, s);
   if (st) return st;

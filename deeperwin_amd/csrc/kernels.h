@@ -57,8 +57,11 @@ int lap_electrons_per_tile(int Cp, GemmPrec prec);
 void gemm_value16(const float* X, int ldx, const float* Bt, int K, const float* V, int ldv, int vsplit, long vstride,
                   float* out, int ldo, int permute, const Shape& sh, hipStream_t s);
 
+// Layer-0 input rows: the el-ion features in columns [0, 4 I); the columns [tail0, tail0 + tailw) are left
+// to the 2-el kernels for the real channel rows; every other column and padding row is zero.
 template <typename T>
-void launch_features0(const float* r, const float* R, T* X0, int Kx0, const Shape& sh, hipStream_t s);
+void launch_features0(const float* r, const float* R, T* X0, int Kx0, int tail0, int tailw, const Shape& sh,
+                      hipStream_t s);
 template <typename T>
 void launch_twoel(const float* r, T* const* X, const int* Kx, const int* Kh, const T* W2e, const T* b2e,
                   const Shape& sh, hipStream_t s);
@@ -104,6 +107,24 @@ int clip_max_walkers();
 void launch_clip_energies(const float* e, int n, int tanh_clip, int median, int mae, double clip_by,
                           int from_previous, const double* state_in, float* weights, float* eclip, double* stats,
                           hipStream_t s);
+// ---- dpe4 embedding (csrc/dpe4.hip) ----
+constexpr int MAXL_DPE4 = 4;  // iterations
+// Pair streams: wm[l][b][i][j][ch][f] (ch: value, d/dd (3), total Laplacian in the Laplacian pass).
+// Wp [2L-1][2][32][32], bp [2L-1][2][32]: the maps of iterations 0..L-1, then the stream layers.
+template <typename T>
+void launch_dpe4_pairs(const float* r, const T* Wp, const T* bp, T* wm, const Shape& sh, hipStream_t s);
+// El-ion stream: e_ei of every iteration into X[l] columns tail[l] + 32 ...; him: scratch [L][I][32].
+template <typename T>
+void launch_dpe4_elion(const float* r, const float* R, const float* Z, const T* Wei, const T* bei, const T* Whim,
+                       const T* bhim, const T* table, int n_types, int z_min, T* him, T* const* X, const int* Kx,
+                       const int* tail, const Shape& sh, hipStream_t s);
+// hm [B n Cp][32] = tanh rule of (hm + bias) over each electron's channel rows, in place.
+template <typename T>
+void launch_dpe4_tanh(T* hm, const T* bias, const Shape& sh, hipStream_t s);
+// e_ee of one iteration from wm (that iteration's slice) and hm into X columns col0 .. col0 + 31.
+template <typename T>
+void launch_dpe4_conv(const T* wm, const T* hm, T* X, int Kx, int col0, const Shape& sh, hipStream_t s);
+
 void launch_gather_walkers(const float* r, const int32_t* idx, int count, int n, float* out, hipStream_t s);
 void launch_scatter_values(const float* v, const int32_t* idx, int count, float* out, hipStream_t s);
 void launch_propose(const float* r, const float* normals, const void* chain, float* r_prop, int count, hipStream_t s);

@@ -30,7 +30,7 @@ __device__ __forceinline__ double dexp(double x) { return exp(x); }
 // ------------------------------------------------------------------------------------------------
 template <typename T>
 __global__ void k_features0(const float* __restrict__ r, const float* __restrict__ R, T* __restrict__ X0, int Kx0,
-                            Shape sh) {
+                            int tail0, int tailw, Shape sh) {
   // one thread per (electron row bi, column): the distance to the column's ion is formed once and the
   // column is written for every channel row (consecutive threads write consecutive columns of a row)
   const long total = (long)sh.B * sh.n * Kx0;
@@ -42,8 +42,9 @@ __global__ void k_features0(const float* __restrict__ r, const float* __restrict
   const int I4 = 4 * sh.I, C = sh.C, Cp = sh.Cp;
   T* out = X0 + bi * Cp * (long)Kx0 + col;
   if (col >= I4) {
-    // pair-mean columns are written by k_twoel for the real channels; padding columns are zero
-    for (int ch = (col < I4 + 8) ? C : 0; ch < Cp; ++ch) out[(long)ch * Kx0] = T(0);
+    // the columns [tail0, tail0 + tailw) are written by the 2-el kernels for the real channels; padding
+    // columns and rows are zero
+    for (int ch = (col >= tail0 && col < tail0 + tailw) ? C : 0; ch < Cp; ++ch) out[(long)ch * Kx0] = T(0);
     return;
   }
   const int ion = col >> 2, f = col & 3;
@@ -67,12 +68,15 @@ __global__ void k_features0(const float* __restrict__ r, const float* __restrict
 }
 
 template <typename T>
-void launch_features0(const float* r, const float* R, T* X0, int Kx0, const Shape& sh, hipStream_t s) {
+void launch_features0(const float* r, const float* R, T* X0, int Kx0, int tail0, int tailw, const Shape& sh,
+                      hipStream_t s) {
   const long total = (long)sh.B * sh.n * Kx0;
-  hipLaunchKernelGGL(k_features0<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, r, R, X0, Kx0, sh);
+  hipLaunchKernelGGL(k_features0<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, r, R, X0, Kx0, tail0,
+                     tailw, sh);
 }
-template void launch_features0<float>(const float*, const float*, float*, int, const Shape&, hipStream_t);
-template void launch_features0<double>(const float*, const float*, double*, int, const Shape&, hipStream_t);
+template void launch_features0<float>(const float*, const float*, float*, int, int, int, const Shape&, hipStream_t);
+template void launch_features0<double>(const float*, const float*, double*, int, int, int, const Shape&,
+                                       hipStream_t);
 
 // ------------------------------------------------------------------------------------------------
 // Fused two-electron stream.  One workgroup per (walker b, group of 8 electrons i); 256 threads =

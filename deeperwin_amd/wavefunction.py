@@ -32,14 +32,15 @@ def _as_device(x, dtype=None):
 class NativeLogPsiSquared:
     """Callable with the reference signature; holds one native model per (shape, params) it has seen."""
 
-    def __init__(self, precision: str = "fp32", fp64_cond_threshold: float = 0.0, chunk: int = 0):
+    def __init__(self, precision: str = "fp32", fp64_cond_threshold: float = 0.0, chunk: int = 0, z_min: int = 1):
         self.precision = precision
         self.fp64_cond_threshold = fp64_cond_threshold
         self.chunk = chunk
+        self.z_min = z_min  # dpe4: Z of the ion embedding table's first row
         self._models: Dict[ModelShape, NativeModel] = {}
 
     def model_for(self, params: Dict, n_up: int, n_dn: int) -> NativeModel:
-        shape = infer_shape(params, n_up + n_dn, n_up)
+        shape = infer_shape(params, n_up + n_dn, n_up, self.z_min)
         m = self._models.get(shape)
         if m is None:
             m = NativeModel(shape, self.precision, self.fp64_cond_threshold, self.chunk)
@@ -69,8 +70,9 @@ def build_log_psi_squared(config: ModelConfigFermiNet, physical_config: Physical
                           fp64_cond_threshold: float = 0.0):
     """Build the native wavefunction and random-initialise its parameters from ``rng_seed``.
 
-    Mirrors model/wavefunction.py:262-299; baseline (CASSCF) orbitals, TAOs and analytic envelope
-    initialisation are not part of the FermiNet-style model and are rejected by ``config.validate``.
+    Mirrors model/wavefunction.py:262-299 for the FermiNet-style model (ModelConfigFermiNet) and the
+    reference's default dpe4 model (ModelConfigDeepErwin4); baseline (CASSCF) orbitals, TAOs and analytic
+    envelope initialisation are not part of either and are rejected by ``config.validate``.
     """
     config.validate()
     if isinstance(physical_config, (list, tuple)):
@@ -79,12 +81,18 @@ def build_log_psi_squared(config: ModelConfigFermiNet, physical_config: Physical
         raise NotImplementedError("baseline orbitals are not used by the FermiNet-style model")
     phys = physical_config
     emb = config.embedding
+    z_min, types = 1, 0
+    if emb.name == "dpe4":  # WavefunctionDefinition (model/wavefunction.py:306-326)
+        z_min = int(min(phys.Z)) if getattr(config, "Z_min", None) is None else int(config.Z_min)
+        z_max = int(max(phys.Z)) if getattr(config, "Z_max", None) is None else int(config.Z_max)
+        types = z_max - z_min + 1
     shape = ModelShape(n_el=phys.n_electrons, n_up=phys.n_up, n_ion=phys.n_ions,
                        n_det=config.orbitals.n_determinants, width1=emb.n_hidden_one_el,
-                       width2=emb.n_hidden_two_el, n_iter=emb.n_iterations)
+                       width2=emb.n_hidden_two_el, n_iter=emb.n_iterations, embedding=emb.name, z_min=z_min,
+                       n_ion_types=types)
     fixed_params = fixed_params or dict(input={}, baseline_energies=dict(E_ref=phys.E_ref))
     params = init_params(shape, rng_seed)
-    log_psi_sqr = NativeLogPsiSquared(precision=precision, fp64_cond_threshold=fp64_cond_threshold)
+    log_psi_sqr = NativeLogPsiSquared(precision=precision, fp64_cond_threshold=fp64_cond_threshold, z_min=z_min)
 
     def get_slater_mat(*args, **kwargs):
         raise NotImplementedError("Slater matrices are internal to the fused native path")

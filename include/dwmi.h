@@ -58,7 +58,20 @@ typedef struct {
   int32_t n_iter;  /* FermiNet iterations (4)                    */
   int32_t precision;       /* DW_FP32 / DW_BF16X3 / DW_BF16 / DW_FP64 */
   float fp64_cond_threshold; /* walker conditioning above which dw_local_energy recomputes in FP64 (<=0: never) */
+  int32_t embedding;       /* DW_EMBEDDING_FERMINET (0) or DW_EMBEDDING_DPE4 (1)                      */
+  int32_t z_min;           /* dpe4: nuclear charge of the ion embedding table's first row (Z_min)      */
+  int32_t n_ion_types;     /* dpe4: rows of the ion embedding table (Z_max - Z_min + 1)                */
 } dw_model_desc;
+
+/* Embeddings.  FERMINET: the FermiNet-style model (ferminet_embedding.py:207-267 with the FermiNet flags).
+ * DPE4: the reference's default model (ModelConfigDeepErwin4, configuration.py:875-891, 422-441, 666-674):
+ * same/diff-spin pair streams started from the el-el distance, an el-ion stream, SchNet-style convolutions
+ * e_ee[i] = sum_j w_same|diff(pair[i, j]) * tanh(h W_map + b)[j] and e_ei[i] = sum_I el_ion[i, I] *
+ * tanh(h_ion W + b)[I] with h_ion = embeddings[Z_I - Z_min] (ferminet_embedding.py:52-188,
+ * input_features.py:176-183); its leaves are listed at dw_model_set_param.  width2 (= emb_dim) must be 32.
+ * The parameter gradient (dw_param_grad) covers FERMINET only. */
+#define DW_EMBEDDING_FERMINET 0
+#define DW_EMBEDDING_DPE4 1
 
 /* Create a model for one (n_el, n_up, n_ion) shape.  Replaces build_log_psi_squared's model
  * construction (model/wavefunction.py:262-299). */
@@ -66,8 +79,10 @@ dw_status dw_model_create(const dw_model_desc* desc, dw_model** out);
 dw_status dw_model_destroy(dw_model* m);
 
 /* Upload one parameter leaf from HOST memory (float32, row-major).  `module_path` is the haiku module
- * path and `leaf` the leaf name (w, b, alpha_up, alpha_dn, weights_up, weights_dn); paths are matched
- * by suffix, so 'wf/ferminet_embedding/h_el_0/linear_0' and '.../~/h_el_0/linear_0' are both accepted.
+ * path and `leaf` the leaf name (w, b, alpha_up, alpha_dn, weights_up, weights_dn, embeddings); paths are
+ * matched by suffix, so 'wf/ferminet_embedding/h_el_0/linear_0' and '.../~/h_el_0/linear_0' are both
+ * accepted.  dpe4 leaves: symm_features_{l}/convolutional_features/{w_same,w_diff,h_map,h_ion_map}/linear_0,
+ * h_same_{l}/linear_0, h_diff_{l}/linear_0, h_el_ion_{l}/linear_0 (w, b) and input/h_ion (embeddings).
  * Replaces handing `params` (a haiku dict) to log_psi_sqr (model/wavefunction.py:293; leaves at
  * mlp.py:57-59, envelope_orbitals.py:101-104). */
 dw_status dw_model_set_param(dw_model* m, const char* module_path, const char* leaf, const float* host_data,

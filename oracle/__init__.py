@@ -69,6 +69,17 @@ PINNING = {
     "hamiltonian.mean_and_var": [
         "test_oracle_physics.py::test_mean_and_var_fsum (math.fsum, NaN dropped, inf kept)",
     ],
+    "wavefunction.dpe4_embedding": [
+        "test_oracle_dpe4.py::test_dpe4_matches_scalar_restatement (scalar loops over the reference's same/diff "
+        "lists and block reshapes, Leibniz determinants)",
+        "test_oracle_dpe4.py::test_planted_mistakes_fail (same/diff swapped, no Z_min offset, self pair dropped)",
+    ],
+    "wavefunction.ion_embedding": [
+        "test_oracle_dpe4.py::test_dpe4_matches_scalar_restatement",
+    ],
+    "hamiltonian.kinetic_energy (dpe4)": [
+        "test_oracle_dpe4.py::test_dpe4_kinetic_energy_finite_differences",
+    ],
     "mcmc.mcmc_steps": [
         "test_oracle_mcmc.py::test_hand_worked_chain (forced accept at the old age, old-rate stepsize update)",
         "test_oracle_mcmc.py::test_stationary_distribution_hydrogen_1s",

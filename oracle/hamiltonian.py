@@ -40,15 +40,24 @@ def potential_energy(r: torch.Tensor, R: torch.Tensor, Z: torch.Tensor) -> torch
     return e_ee + e_ei + e_II
 
 
+def _charges(Z):
+    """Nuclear charges as a NumPy array, read before entering torch.func transforms (the dpe4 model's ion
+    embedding indexes its table with them)."""
+    if Z is None:
+        return None
+    return np.asarray(Z.detach().cpu().numpy() if isinstance(Z, torch.Tensor) else Z)
+
+
 def kinetic_energy(params: Dict, n_up: int, n_dn: int, r: torch.Tensor, R: torch.Tensor,
-                   dtype=torch.float64) -> torch.Tensor:
+                   dtype=torch.float64, Z=None) -> torch.Tensor:
     """Per-walker kinetic energy for r [B, n, 3] by forward-over-reverse Hessian-vector products."""
     n = n_up + n_dn
     R = R.to(dtype)
+    Z = _charges(Z)
 
     def f(x):  # x: [3n] -> log psi^2 of one walker (evaluated as a batch of one: in this PyTorch build the
         # float32 forward-over-reverse of slogdet fails for un-batched inputs)
-        return wf.log_psi_sqr(params, n_up, n_dn, x.reshape(1, n, 3), R, dtype=dtype)[1][0]
+        return wf.log_psi_sqr(params, n_up, n_dn, x.reshape(1, n, 3), R, Z, dtype=dtype)[1][0]
 
     grad_f = torch.func.grad(f)
     eye = torch.eye(3 * n, dtype=dtype)
@@ -64,7 +73,7 @@ def kinetic_energy(params: Dict, n_up: int, n_dn: int, r: torch.Tensor, R: torch
 
 
 def kinetic_energy_vmapped(params: Dict, n_up: int, n_dn: int, r: torch.Tensor, R: torch.Tensor,
-                           dtype=torch.float64) -> torch.Tensor:
+                           dtype=torch.float64, Z=None) -> torch.Tensor:
     """kinetic_energy with the walkers of r [B, n, 3] batched by vmap, as folx.batched_vmap batches a chunk
     in the reference (hamiltonian.py:280).  Only for models with two or more determinants: with a single
     determinant per walker the vmapped forward-over-reverse of slogdet is wrong in this PyTorch build (see
@@ -73,11 +82,12 @@ def kinetic_energy_vmapped(params: Dict, n_up: int, n_dn: int, r: torch.Tensor, 
 
     n = n_up + n_dn
     if infer_shape(params, n, n_up).n_det < 2:
-        return kinetic_energy(params, n_up, n_dn, r, R, dtype=dtype)
+        return kinetic_energy(params, n_up, n_dn, r, R, dtype=dtype, Z=Z)
     R = R.to(dtype)
+    Z = _charges(Z)
 
     def f(x):
-        return wf.log_psi_sqr(params, n_up, n_dn, x.reshape(1, n, 3), R, dtype=dtype)[1][0]
+        return wf.log_psi_sqr(params, n_up, n_dn, x.reshape(1, n, 3), R, Z, dtype=dtype)[1][0]
 
     grad_f = torch.func.grad(f)
     eye = torch.eye(3 * n, dtype=dtype)
@@ -101,7 +111,7 @@ def local_energy(params: Dict, n_up: int, n_dn: int, r, R, Z, max_batch_size: in
     for s in range(0, r.shape[0], max_batch_size):
         rc = r[s:s + max_batch_size]
         kin = kinetic_energy_vmapped if vmapped else kinetic_energy
-        out.append(kin(params, n_up, n_dn, rc, R, dtype=dtype) + potential_energy(rc, R.to(dtype), Z))
+        out.append(kin(params, n_up, n_dn, rc, R, dtype=dtype, Z=Z) + potential_energy(rc, R.to(dtype), Z))
     return torch.cat(out)
 
 

@@ -92,6 +92,68 @@ def ferminet_embedding(params: Dict, h: torch.Tensor, h_ee: torch.Tensor, n_up: 
     return h
 
 
+def _is_dpe4(params: Dict) -> bool:
+    try:
+        _leaf(params, "symm_features_0/convolutional_features/h_map/linear_0", "w", torch.float64)
+        return True
+    except KeyError:
+        return False
+
+
+def _dense(params: Dict, module: str, x: torch.Tensor) -> torch.Tensor:
+    """MLP([width]) of one layer without residual: tanh(x W + b) (model/mlp.py:45-68, residual off)."""
+    dt = x.dtype
+    return torch.tanh(x @ _leaf(params, f"{module}/linear_0", "w", dt) + _leaf(params, f"{module}/linear_0", "b", dt))
+
+
+def ion_embedding(params: Dict, Z, z_min: int, dtype) -> torch.Tensor:
+    """hk.Embed lookup of Z - Z_min in the 'h_ion' table (input_features.py:176-183)."""
+    table = _leaf(params, "input/h_ion", "embeddings", dtype)
+    idx = torch.as_tensor(np.asarray(Z)).round().long() - int(z_min)
+    return table[idx]
+
+
+def dpe4_embedding(params: Dict, h: torch.Tensor, r: torch.Tensor, R: torch.Tensor, Z, z_min: int, n_up: int,
+                   n_iter: int) -> torch.Tensor:
+    """FermiNetEmbedding with the dpe4 flags (configuration.py:422-441; ferminet_embedding.py:97-188,
+    207-267): per iteration i the 1-el input is [h, mean_up h, mean_dn h, e_ee, e_ei] with the SchNet-style
+    convolutions e_ee[i] = sum_j w_{s(i,j)}(h2[i, j]) * tanh(h W_map + b)[j] (w_same for equal spins, w_diff
+    otherwise, the self pair included, sum aggregation) and e_ei[i] = sum_I h_ei[i, I] * tanh(h_ion W + b)[I];
+    then h = tanh(. W + b); the same/diff pair streams (started from the el-el distance alone,
+    use_el_el_differences=False) and the el-ion stream (started from [dist, diff]) take one residual tanh
+    layer each between iterations.  Computing both pair maps on every pair and selecting by spin equals the
+    reference's split into same/diff lists, because both maps act pair by pair."""
+    dt = h.dtype
+    n = h.shape[-2]
+    _, _, x_ei = el_ion_features(r, R)                           # [..., n, I, 4]
+    _, dist_ee, _ = el_el_features(r)
+    x_ee = dist_ee[..., None]                                   # [..., n, n, 1]
+    spin = torch.arange(n) >= n_up
+    same = (spin[:, None] == spin[None, :])[..., None]          # [n, n, 1]
+    h_ion = ion_embedding(params, Z, z_min, dt)                 # [I, 32]
+    for i in range(n_iter):
+        conv = f"symm_features_{i}/convolutional_features"
+        up = h[..., :n_up, :].mean(dim=-2, keepdim=True).expand(*h.shape[:-2], n, h.shape[-1])
+        dn = h[..., n_up:, :].mean(dim=-2, keepdim=True).expand(*h.shape[:-2], n, h.shape[-1])
+        w = torch.where(same, _dense(params, f"{conv}/w_same", x_ee), _dense(params, f"{conv}/w_diff", x_ee))
+        hm = _dense(params, f"{conv}/h_map", h)                 # [..., n, E]
+        e_ee = (w * hm[..., None, :, :]).sum(dim=-2)            # sum over partners j
+        him = _dense(params, f"{conv}/h_ion_map", h_ion)        # [I, g]
+        e_ei = (x_ei * him).sum(dim=-2)                         # sum over ions
+        f = torch.cat([h, up, dn, e_ee, e_ei], dim=-1)
+        h = mlp_layer(f, _leaf(params, f"h_el_{i}/linear_0", "w", dt), _leaf(params, f"h_el_{i}/linear_0", "b", dt))
+        if i == n_iter - 1:
+            break
+        x_ee = torch.where(same,
+                           mlp_layer(x_ee, _leaf(params, f"h_same_{i}/linear_0", "w", dt),
+                                     _leaf(params, f"h_same_{i}/linear_0", "b", dt)),
+                           mlp_layer(x_ee, _leaf(params, f"h_diff_{i}/linear_0", "w", dt),
+                                     _leaf(params, f"h_diff_{i}/linear_0", "b", dt)))
+        x_ei = mlp_layer(x_ei, _leaf(params, f"h_el_ion_{i}/linear_0", "w", dt),
+                         _leaf(params, f"h_el_ion_{i}/linear_0", "b", dt))
+    return h
+
+
 def envelope_orbitals(params: Dict, h: torch.Tensor, dist_ei: torch.Tensor, n_up: int, n_dn: int, n_det: int):
     """Full-determinant envelope orbitals (model/orbitals/envelope_orbitals.py:39-127):
     bf_s = h_s W_s (no bias) reshaped to [D, n_s, n]; env_s[i, c] = sum_I w_s[I, c] exp(-softplus(alpha_s[I, c])
@@ -124,35 +186,7 @@ def sum_of_determinants(mo_up: torch.Tensor, mo_dn: torch.Tensor):
     return phase, log_psi_sqr
 
 
-def log_psi_sqr(params: Dict, n_up: int, n_dn: int, r, R, Z=None, fixed_params=None, n_det: int = None,
-                dtype=torch.float64) -> Tuple[torch.Tensor, torch.Tensor]:
-    """Wavefunction.__call__ (model/wavefunction.py:118-134) for the FermiNet-style model:
-    InputPreprocessor (input_features.py:112-303: el-ion features concatenated ion-major into h_one,
-    el-el features, ion-ion features unused) -> FermiNetEmbedding -> EnvelopeOrbitals -> sum of dets.
-    Jastrow and el-el cusp are off for this model (configuration.py:913-923)."""
-    r = torch.as_tensor(r, dtype=dtype) if not isinstance(r, torch.Tensor) else r.to(dtype)
-    R = torch.as_tensor(np.asarray(R), dtype=dtype) if not isinstance(R, torch.Tensor) else R.to(dtype)
-    n = n_up + n_dn
-    n_iter = 0
-    while True:
-        try:
-            _leaf(params, f"h_el_{n_iter}/linear_0", "b", dtype)
-        except KeyError:
-            break
-        n_iter += 1
-    if n_det is None:
-        n_det = _leaf(params, "bf_up/linear_0", "w", dtype).shape[1] // n
-    _, dist_ei, f_ei = el_ion_features(r, R)
-    _, _, f_ee = el_el_features(r)
-    h = f_ei.reshape(*f_ei.shape[:-2], -1)           # [..., n, 4 * n_ion], ion-major
-    h = ferminet_embedding(params, h, f_ee, n_up, n_iter)
-    mo_up, mo_dn = envelope_orbitals(params, h, dist_ei, n_up, n_dn, n_det)
-    return sum_of_determinants(mo_up, mo_dn)
-
-
-def slater_matrices(params: Dict, n_up: int, n_dn: int, r, R, n_det: int = None, dtype=torch.float64):
-    """Stacked [..., D, n, n] orbital matrices (Wavefunction.get_slater_matrices,
-    model/wavefunction.py:136-143, with the up and down rows concatenated as in wavefunction.py:66-68)."""
+def _orbitals(params: Dict, n_up: int, n_dn: int, r, R, Z, z_min, n_det, dtype):
     r = torch.as_tensor(np.asarray(r), dtype=dtype) if not isinstance(r, torch.Tensor) else r.to(dtype)
     R = torch.as_tensor(np.asarray(R), dtype=dtype) if not isinstance(R, torch.Tensor) else R.to(dtype)
     n = n_up + n_dn
@@ -166,17 +200,43 @@ def slater_matrices(params: Dict, n_up: int, n_dn: int, r, R, n_det: int = None,
     if n_det is None:
         n_det = _leaf(params, "bf_up/linear_0", "w", dtype).shape[1] // n
     _, dist_ei, f_ei = el_ion_features(r, R)
-    _, _, f_ee = el_el_features(r)
-    h = ferminet_embedding(params, f_ei.reshape(*f_ei.shape[:-2], -1), f_ee, n_up, n_iter)
-    mo_up, mo_dn = envelope_orbitals(params, h, dist_ei, n_up, n_dn, n_det)
+    h = f_ei.reshape(*f_ei.shape[:-2], -1)           # [..., n, 4 * n_ion], ion-major
+    if _is_dpe4(params):
+        if Z is None:
+            raise ValueError("the dpe4 model needs the nuclear charges Z (ion embedding)")
+        if z_min is None:
+            z_min = int(np.min(np.asarray(Z)))     # Z_min defaults to min(Z) (model/wavefunction.py:318-324)
+        h = dpe4_embedding(params, h, r, R, Z, z_min, n_up, n_iter)
+    else:
+        _, _, f_ee = el_el_features(r)
+        h = ferminet_embedding(params, h, f_ee, n_up, n_iter)
+    return envelope_orbitals(params, h, dist_ei, n_up, n_dn, n_det)
+
+
+def log_psi_sqr(params: Dict, n_up: int, n_dn: int, r, R, Z=None, fixed_params=None, n_det: int = None,
+                dtype=torch.float64, z_min: int = None) -> Tuple[torch.Tensor, torch.Tensor]:
+    """Wavefunction.__call__ (model/wavefunction.py:118-134) for the FermiNet-style and dpe4 models:
+    InputPreprocessor (input_features.py:112-303: el-ion features concatenated ion-major into h_one,
+    el-el features, ion-ion features unused) -> FermiNetEmbedding -> EnvelopeOrbitals -> sum of dets.
+    Jastrow and el-el cusp are off for both models (configuration.py:875-891, 913-923).  Z (and z_min, by
+    default min(Z)) is read by the dpe4 model's ion embedding only."""
+    mo_up, mo_dn = _orbitals(params, n_up, n_dn, r, R, Z, z_min, n_det, dtype)
+    return sum_of_determinants(mo_up, mo_dn)
+
+
+def slater_matrices(params: Dict, n_up: int, n_dn: int, r, R, n_det: int = None, dtype=torch.float64, Z=None,
+                    z_min: int = None):
+    """Stacked [..., D, n, n] orbital matrices (Wavefunction.get_slater_matrices,
+    model/wavefunction.py:136-143, with the up and down rows concatenated as in wavefunction.py:66-68)."""
+    mo_up, mo_dn = _orbitals(params, n_up, n_dn, r, R, Z, z_min, n_det, dtype)
     return torch.cat([mo_up, mo_dn], dim=-2)
 
 
-def conditioning(params: Dict, n_up: int, n_dn: int, r, R) -> np.ndarray:
+def conditioning(params: Dict, n_up: int, n_dn: int, r, R, Z=None) -> np.ndarray:
     """Per-walker sensitivity of log|psi| to relative perturbations of the orbital matrices:
     kappa = max_d cond_2(A_d) * sum_d |s_d e^{l_d - m}| / |sum_d s_d e^{l_d - m}| (FP64).
     Rounding at unit roundoff u in the network perturbs log|psi|^2 by O(u kappa)."""
-    mo = slater_matrices(params, n_up, n_dn, r, R)
+    mo = slater_matrices(params, n_up, n_dn, r, R, Z=Z)
     sign, logdet = torch.linalg.slogdet(mo)
     m = torch.amax(logdet, dim=-1, keepdim=True)
     e = sign * torch.exp(logdet - m)

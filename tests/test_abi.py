@@ -46,7 +46,7 @@ def test_model_create_validates_descriptor():
     assert lib.dw_model_ready(h) == 0
     assert lib.dw_model_destroy(h) == 0
     for bad in (dict(n_el=65), dict(n_up=0), dict(n_up=14), dict(width2=16), dict(width1=200), dict(n_iter=5),
-                dict(precision=7)):
+                dict(precision=7), dict(embedding=2), dict(embedding=1, n_ion_types=0)):
         h = ctypes.c_void_p()
         assert lib.dw_model_create(ctypes.byref(_desc(**bad)), ctypes.byref(h)) == 1, bad
         assert lib.dw_last_error().decode()
@@ -70,6 +70,29 @@ def test_parameter_ingestion_and_errors():
     assert st == 1 and "unknown" in m.lib.dw_last_error().decode()
     with pytest.raises(_native.DwError):
         m.set_params({"wf/ferminet_embedding/h_el_0/linear_0": {"w": bad}})
+
+
+def test_dpe4_parameter_ingestion():
+    """The dpe4 model takes the reference's dpe4 leaves (conv maps, same/diff and el-ion streams, the ion
+    table) and refuses the FermiNet 2-el leaves; the parameter gradient refuses the dpe4 model on the host."""
+    shape = ModelShape(14, 7, 2, 4, embedding="dpe4", z_min=7, n_ion_types=1)
+    m = _native.NativeModel(shape)
+    params = init_params(shape, 3)
+    m.set_params({k.replace("wf/", "wf/~/"): v for k, v in params.items()})
+    assert m.lib.dw_model_ready(m.handle) == 1
+    w = np.zeros((4, 32), np.float32)
+    shp = (ctypes.c_int64 * 2)(*w.shape)
+    st = m.lib.dw_model_set_param(m.handle, b"wf/ferminet_embedding/h_el_el_0/linear_0", b"w",
+                                  w.ctypes.data_as(ctypes.c_void_p), shp, 2)
+    assert st == 1 and "unknown" in m.lib.dw_last_error().decode()
+    m2 = _native.NativeModel(shape)
+    incomplete = dict(params)
+    incomplete.pop("wf/input/h_ion")
+    with pytest.raises(_native.DwError, match="not every parameter"):
+        m2.set_params(incomplete)
+    fake = ctypes.c_void_p(256)
+    assert m.lib.dw_param_grad(m.handle, fake, 8, fake, fake, fake, fake, None, None) == 1
+    assert "dpe4" in m.lib.dw_last_error().decode()
 
 
 def test_missing_leaf_is_reported():

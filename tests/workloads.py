@@ -54,3 +54,21 @@ def system(name: str, n_det: int = None, seed: int = 0):
 
 def walkers(R, mapping, B: int, seed: int = 0):
     return init_walkers_gaussian(np.random.default_rng(seed), B, R, mapping).astype(np.float32)
+
+
+def system_dpe4(name: str, n_det: int = 4, seed: int = 0, bias_scale: float = 0.3):
+    """The geometries of `system` with the dpe4 model (Z_min / Z_max of the molecule fix the ion table).
+    Biases are drawn from U(-bias_scale, bias_scale) so that every term of the convolutions (the self pair
+    included) is non-zero."""
+    shape, _, R, Z, mapping = system(name, n_det, seed)
+    z = np.asarray(Z).astype(int)
+    from deeperwin_amd.params import ModelShape
+
+    shape = ModelShape(shape.n_el, shape.n_up, shape.n_ion, shape.n_det, embedding="dpe4", z_min=int(z.min()),
+                       n_ion_types=int(z.max() - z.min() + 1))
+    params = init_params(shape, seed)
+    rng = np.random.default_rng(seed + 1000)
+    for leaves in params.values():
+        if "b" in leaves:
+            leaves["b"] = rng.uniform(-bias_scale, bias_scale, leaves["b"].shape).astype(np.float32)
+    return shape, params, R, Z, mapping
