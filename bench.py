@@ -46,7 +46,8 @@ def parse():
     ap.add_argument("--cpu-walkers", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--workload", default="n2", choices=["n2", "c4h2-precision", "serving", "batch64", "copy", "grad"],
+    ap.add_argument("--workload", default="n2", choices=["n2", "c4h2-precision", "serving", "batch64", "copy", "grad",
+                                                          "dpe4"],
                     help="n2: the headline line; c4h2-precision: the precision-form study (E_loc only); "
                          "serving: Poisson request stream with continuous batching; batch64: 64 geometries "
                          "under shared weights in one graph launch per unit; copy: a plain device copy "
@@ -480,7 +481,7 @@ def cpu_baseline(params, n_up, n_dn, r0, R, Z, n_steps, k, reps=3, chunk=64):
     n = n_up + n_dn
 
     def f(x):
-        return owf.log_psi_sqr(params, n_up, n_dn, _t.tensor(x), R, dtype=_t.float32)[1].numpy()
+        return owf.log_psi_sqr(params, n_up, n_dn, _t.tensor(x), R, Z, dtype=_t.float32)[1].numpy()
 
     def run(m):
         rc = r0[:m].astype(np.float32)
@@ -507,6 +508,97 @@ def cpu_baseline(params, n_up, n_dn, r0, R, Z, n_steps, k, reps=3, chunk=64):
             "baseline_only": True}
 
 
+def dpe4_bench(args):
+    """The N2 unit (walkers x [20 MCMC steps + E_loc]) with the reference's default dpe4 model: same/diff
+    pair streams, el-ion stream and SchNet-style convolutions (configuration.py:875-891), 16 dets, seed 1."""
+    import torch
+
+    from deeperwin_amd._native import TIMER_LOCAL_ENERGY, NativeModel
+    from deeperwin_amd.mcmc import make_chain
+    from deeperwin_amd.params import ModelShape, init_params
+    from deeperwin_amd.physics import generate_el_ion_mapping, geometry_n2, init_walkers_gaussian
+
+    R, Z = geometry_n2()
+    n, u, D, B, S = 14, 7, 16, args.walkers, args.mcmc_steps
+    shape = ModelShape(n_el=n, n_up=u, n_ion=2, n_det=D, embedding="dpe4", z_min=7, n_ion_types=1)
+    params = init_params(shape, seed=1)
+    r0 = init_walkers_gaussian(np.random.default_rng(1), B, R, generate_el_ion_mapping(R, Z, n, u))
+    dev = "cuda"
+    model = NativeModel(shape, args.precision)
+    model.set_params(params)
+    r = torch.as_tensor(r0, dtype=torch.float32, device=dev).contiguous()
+    Rd = torch.as_tensor(R, dtype=torch.float32, device=dev)
+    Zd = torch.as_tensor(Z, dtype=torch.float32, device=dev)
+    lp = torch.full((B,), -1000.0, device=dev)
+    age = torch.zeros(B, dtype=torch.int32, device=dev)
+    chain = make_chain(0.02)
+    e = torch.empty(B, device=dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(1)
+    events = []
+
+    def step(record=False):
+        normals = torch.randn((S, B, n, 3), generator=gen, device=dev)
+        uniforms = torch.rand((S, B), generator=gen, device=dev)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        model.mcmc_steps(r, Rd, Zd, lp, age, normals, uniforms, chain, 100, 100, 0.5, 1e-2, 1.0)
+        b.record()
+        if record:
+            events.append((a, b))
+        model.local_energy(r, Rd, Zd, e)
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    model.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(record=True)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    lap_ms, lap_n = model.timer(TIMER_LOCAL_ENERGY)
+    model.set_timing(False)
+    mcmc_ms = float(np.mean([a.elapsed_time(b) for a, b in events]))
+
+    def eloc_with(m):
+        out_e = torch.empty(B, device=dev)
+        m.local_energy(r, Rd, Zd, out_e)
+        torch.cuda.synchronize()
+        return out_e.double().cpu().numpy()
+
+    m64 = NativeModel(shape, "fp64", chunk=1024)
+    m64.set_params(params)
+    e64, ex = eloc_with(m64), eloc_with(model)
+    ok = np.isfinite(e64) & np.isfinite(ex)
+    out = {
+        "metric": METRIC, "value": round(B / dt, 1), "unit": "walker local-energies/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
+        "config": {"workload": f"N2 with the dpe4 model (same/diff pair streams, el-ion stream, SchNet-style "
+                               f"convolutions, 4 iterations 256/32), 16 full dets, seed 1; {B} walkers; {S} MCMC "
+                               f"steps (stepsize 0.02) + E_loc over {3 * n} directions",
+                   "walkers": B, "mcmc_steps": S, "precision": args.precision},
+        "breakdown": {"mcmc_step_ms": round(mcmc_ms / S, 4),
+                      "local_energy_pass_ms": round(lap_ms / max(lap_n, 1), 4)},
+        "vs_fp64_form": {"dE_mean_mHa": round(float(np.mean(ex[ok]) - np.mean(e64[ok])) * 1e3, 4),
+                         "median_abs_dE": float(np.median(np.abs(ex[ok] - e64[ok]))),
+                         "nonfinite": int((~ok).sum())},
+    }
+    if not args.no_accuracy:
+        from oracle import hamiltonian as oh
+
+        idx = np.linspace(0, B - 1, 8).astype(int)
+        r_fin = r.cpu().numpy().astype(np.float64)
+        e_ref = oh.local_energy(params, u, n - u, r_fin[idx], R, Z).numpy()
+        out["accuracy"] = {"sample_walkers": 8, "vs": "FP64 CPU oracle on the same walkers",
+                           "max_abs_dE_fp32": float(np.max(np.abs(ex[idx] - e_ref))),
+                           "max_abs_dE_fp64_form": float(np.max(np.abs(e64[idx] - e_ref)))}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(params, u, n - u, r0, R, Z, S, max(8, args.cpu_walkers // 4))
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.workload != "n2" and args.gpus != 1:
@@ -526,6 +618,9 @@ def main():
         return
     if args.workload == "grad":
         grad_bench(args)
+        return
+    if args.workload == "dpe4":
+        dpe4_bench(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (walkers shard across GPUs "

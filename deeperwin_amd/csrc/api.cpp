@@ -576,6 +576,7 @@ dw_status run_pass(dw_model* m, const PassBufs<T>& ws, const float* r, long B, c
   for (int l = 0; l < m->L; ++l) {
     if (m->dpe4) {  // e_ee of this iteration: hm = tanh rule of (X_l[:, h] W_map), then the convolution
       GemmEpiT<T> eh;
+      eh.tile = 0;  // 64 x 64 tiles: the output is only 32 wide
       gemm_t(m, X[l], m->Kx[l], W.WhmT[l].as<T>(), W.WhmT_hi[l], W.WhmT_lo[l], m->Kt0[l], ws.HM, 32, rows, 32,
              m->Kt0[l], eh, s);
       launch_dpe4_tanh<T>(ws.HM, W.bhm[l].as<T>(), sh, s);

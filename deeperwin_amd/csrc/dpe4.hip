@@ -305,83 +305,184 @@ __global__ __launch_bounds__(256) void k_dpe4_tanh(T* __restrict__ hm, const T* 
 // e_ee[i][ch][f] = sum_j w[i, j][f] hm[j][ch][f] plus, in the Laplacian pass, the pair derivatives:
 //   tangent row (e, c):  + (delta_{e j} - delta_{e i}) g_c[i, j] hm[j][0]
 //   Laplacian row:       + L_w[i, j] hm[j][0] + 2 sum_c g_c[i, j] (hm[j][(j, c)] - hm[j][(i, c)])
-// (g = d w / d d, L_w the pair's total Laplacian).  One thread per (walker, electron i, block of RB channel
-// rows, f); blocks of one walker are dealt to one XCD so that its hm rows are shared in that XCD's L2.
+// (g = d w / d d, L_w the pair's total Laplacian).  One workgroup per (walker, block of RB channel rows):
+// the walker's hm rows of the block (n x RB x 32) and its value rows are staged in LDS once, then thread
+// (il, f) forms electrons i = il, il + 8, ... reading each w[i, j] once per block.
 // ------------------------------------------------------------------------------------------------------
 template <bool LAP, int RB, typename T>
 __global__ __launch_bounds__(256) void k_dpe4_conv(const T* __restrict__ wm, const T* __restrict__ hm,
-                                                   T* __restrict__ X, int Kx, int col0, Shape sh, long threads) {
+                                                   T* __restrict__ X, int Kx, int col0, Shape sh) {
   constexpr int NCH = LAP ? 5 : 1;
+  extern __shared__ unsigned char smem_raw[];
+  T* hs = (T*)smem_raw;                         // [n][RB][32]
   const int n = sh.n, C = sh.C, Cp = sh.Cp;
-  const int nRB = (Cp + RB - 1) / RB;
-  // XCD-aware block order: the 8 XCDs take blockIdx round-robin, so XCD x gets the contiguous range
-  // [x per, (x + 1) per) of logical blocks (the grid is a multiple of 8; blocks past the work return)
-  const long per = gridDim.x / 8;
-  const long blk = (long)(blockIdx.x % 8) * per + blockIdx.x / 8;
-  const long t = blk * blockDim.x + threadIdx.x;
-  if (t >= threads) return;
-  const int f = (int)(t % EMB);
-  const long q = t / EMB;
-  const int rbk = (int)(q % nRB);
-  const long bi = q / nRB;
-  const long b = bi / n;
-  const int i = (int)(bi - b * n);
-  const int ch0 = rbk * RB;
-  T acc[RB];
-#pragma unroll
-  for (int k = 0; k < RB; ++k) acc[k] = T(0);
-  const T* wrow = wm + ((long)bi * n * NCH) * EMB + f;        // w[b][i][j][ch][f], j = 0
-  const T* hb = hm + (long)b * n * Cp * EMB + f;              // hm[b][j][ch][f]
+  T* h0s = hs + (size_t)n * RB * EMB;           // [n][32]: value rows
+  const int nRB = LAP ? (Cp + RB - 1) / RB : 1;
+  const long b = blockIdx.x / nRB;
+  const int ch0 = (int)(blockIdx.x % nRB) * RB;
+  const T* hb = hm + (long)b * n * Cp * EMB;
+  for (int q = threadIdx.x; q < n * RB * EMB; q += 256) {
+    const int j = q / (RB * EMB), k = (q / EMB) % RB, f = q % EMB;
+    hs[q] = (ch0 + k < C) ? hb[((long)j * Cp + ch0 + k) * EMB + f] : T(0);
+  }
+  for (int q = threadIdx.x; q < n * EMB; q += 256) h0s[q] = hb[(long)(q / EMB) * Cp * EMB + q % EMB];
+  __syncthreads();
+  const int il = threadIdx.x >> 5, f = threadIdx.x & 31;
   const bool has_lap = LAP && (C - 1) >= ch0 && (C - 1) < ch0 + RB;
-  for (int j = 0; j < n; ++j) {
-    const T* w = wrow + (long)j * NCH * EMB;
-    const T* h = hb + (long)j * Cp * EMB;
-    const T w0 = w[0];
+  for (int i = il; i < n; i += 8) {
+    T acc[RB];
 #pragma unroll
-    for (int k = 0; k < RB; ++k)
-      if (ch0 + k < C) acc[k] += w0 * h[(long)(ch0 + k) * EMB];
-    if constexpr (LAP) {
-      if (j == i) continue;  // the self pair's w is constant
-      const T h0 = h[0];
-      const T g[3] = {w[EMB], w[2 * EMB], w[3 * EMB]};
+    for (int k = 0; k < RB; ++k) acc[k] = T(0);
+    const T* wrow = wm + ((long)(b * n + i) * n * NCH) * EMB + f;
+    for (int j = 0; j < n; ++j) {
+      const T* w = wrow + (long)j * NCH * EMB;
+      const T w0 = w[0];
+      const T* hj = hs + (size_t)j * RB * EMB + f;
 #pragma unroll
-      for (int k = 0; k < RB; ++k) {
-        const int ch = ch0 + k;
-        if (ch >= 1 && ch < C - 1) {
-          const int kk = ch - 1, e = kk / 3, c = kk - 3 * e;
-          if (e == j) acc[k] += g[c] * h0;
-          else if (e == i) acc[k] -= g[c] * h0;
+      for (int k = 0; k < RB; ++k) acc[k] += w0 * hj[k * EMB];
+      if constexpr (LAP) {
+        if (j == i) continue;  // the self pair's w is constant
+        const T h0 = h0s[j * EMB + f];
+        const T g[3] = {w[EMB], w[2 * EMB], w[3 * EMB]};
+#pragma unroll
+        for (int k = 0; k < RB; ++k) {
+          const int ch = ch0 + k;
+          if (ch >= 1 && ch < C - 1) {
+            const int kk = ch - 1, e = kk / 3, c = kk - 3 * e;
+            if (e == j) acc[k] += g[c] * h0;
+            else if (e == i) acc[k] -= g[c] * h0;
+          }
+        }
+        if (has_lap) {
+          const T* hg = hb + (long)j * Cp * EMB + f;
+          T v = w[4 * EMB] * h0;
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+            v += T(2) * g[c] * (hg[(long)(1 + 3 * j + c) * EMB] - hg[(long)(1 + 3 * i + c) * EMB]);
+          acc[C - 1 - ch0] += v;
         }
       }
-      if (has_lap) {
-        T v = w[4 * EMB] * h0;
+    }
+    T* out = X + (long)(b * n + i) * Cp * Kx + col0 + f;
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-          v += T(2) * g[c] * (h[(long)(1 + 3 * j + c) * EMB] - h[(long)(1 + 3 * i + c) * EMB]);
-        acc[C - 1 - ch0] += v;
-      }
+    for (int k = 0; k < RB; ++k)
+      if (ch0 + k < Cp) out[(long)(ch0 + k) * Kx] = acc[k];
+  }
+}
+
+// ------------------------------------------------------------------------------------------------------
+// Value pass of the pair streams: one lane per pair.  The pairs of one spin relation (s = 0 same, 1 diff)
+// are enumerated walker by walker (same: up-up then down-down; diff: up-down then down-up) and the blocks
+// of each relation are contiguous, so a wavefront's weights are uniform (scalar loads).  Each lane keeps
+// its pair's 32 features in registers and writes w_l[b][i][j][f] as 16-byte vectors.
+// ------------------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_dpe4_pairs_value(const float* __restrict__ r, const T* __restrict__ Wp,
+                                                          const T* __restrict__ bp, T* __restrict__ wm, Shape sh,
+                                                          int blocks_same) {
+  const int n = sh.n, u = sh.u, d = sh.n - sh.u, L = sh.L;
+  const int s = (int)blockIdx.x >= blocks_same ? 1 : 0;
+  const long P = s == 0 ? (long)u * u + (long)d * d : 2L * u * d;  // pairs of this relation per walker
+  const long t = (long)(blockIdx.x - (s ? blocks_same : 0)) * blockDim.x + threadIdx.x;
+  if (t >= (long)sh.B * P) return;
+  const long b = t / P;
+  const int q = (int)(t - b * P);
+  int i, j;
+  if (s == 0) {
+    if (q < u * u) {
+      i = q / u;
+      j = q - i * u;
+    } else {
+      const int q2 = q - u * u;
+      i = u + q2 / d;
+      j = u + q2 % d;
+    }
+  } else {
+    if (q < u * d) {
+      i = q / d;
+      j = u + q % d;
+    } else {
+      const int q2 = q - u * d;
+      i = u + q2 / u;
+      j = q2 % u;
     }
   }
-  T* out = X + (long)bi * Cp * Kx + col0 + f;
+  const float* rb = r + b * n * 3;
+  T x[EMB];
 #pragma unroll
-  for (int k = 0; k < RB; ++k)
-    if (ch0 + k < Cp) out[(long)(ch0 + k) * Kx] = (ch0 + k < C) ? acc[k] : T(0);
+  for (int k = 0; k < EMB; ++k) x[k] = T(0);
+  if (i != j) {
+    const T dx = (T)rb[j * 3 + 0] - (T)rb[i * 3 + 0], dy = (T)rb[j * 3 + 1] - (T)rb[i * 3 + 1],
+            dz = (T)rb[j * 3 + 2] - (T)rb[i * 3 + 2];
+    x[0] = sqrt_t(dx * dx + dy * dy + dz * dz);
+  }
+  const long per_l = (long)sh.B * n * n * EMB;
+  T* out = wm + ((long)(b * n + i) * n + j) * EMB;
+  const T rs2 = (T)0.70710678118654752440;
+  for (int l = 0; l < L; ++l) {
+    const T* Wm = Wp + ((size_t)l * 2 + s) * EMB * EMB;
+    const T* Bm = bp + ((size_t)l * 2 + s) * EMB;
+    // the map w_l
+#pragma unroll
+    for (int f0 = 0; f0 < EMB; f0 += 4) {
+      T y4[4];
+#pragma unroll
+      for (int q4 = 0; q4 < 4; ++q4) {
+        const int fo = f0 + q4;
+        T z = Bm[fo];
+        if (l == 0) {
+          z += Wm[fo] * x[0];
+        } else {
+#pragma unroll
+          for (int k = 0; k < EMB; ++k) z += Wm[k * EMB + fo] * x[k];
+        }
+        y4[q4] = tanh_t(z);
+      }
+      if constexpr (sizeof(T) == 4) {
+        *(float4*)(out + l * per_l + f0) = make_float4(y4[0], y4[1], y4[2], y4[3]);
+      } else {
+        *(double2*)(out + l * per_l + f0) = make_double2(y4[0], y4[1]);
+        *(double2*)(out + l * per_l + f0 + 2) = make_double2(y4[2], y4[3]);
+      }
+    }
+    if (l == L - 1) break;
+    // the stream layer h_same / h_diff (residual from the second iteration on)
+    const T* Wh = Wp + ((size_t)(L + l) * 2 + s) * EMB * EMB;
+    const T* Bh = bp + ((size_t)(L + l) * 2 + s) * EMB;
+    T y[EMB];
+#pragma unroll
+    for (int fo = 0; fo < EMB; ++fo) {
+      T z = Bh[fo];
+      if (l == 0) {
+        z += Wh[fo] * x[0];
+      } else {
+#pragma unroll
+        for (int k = 0; k < EMB; ++k) z += Wh[k * EMB + fo] * x[k];
+      }
+      y[fo] = tanh_t(z);
+    }
+#pragma unroll
+    for (int k = 0; k < EMB; ++k) x[k] = (l == 0) ? y[k] : (x[k] + y[k]) * rs2;
+  }
 }
 
 }  // namespace
 
 template <typename T>
 void launch_dpe4_pairs(const float* r, const T* Wp, const T* bp, T* wm, const Shape& sh, hipStream_t s) {
+  if (sh.C == 1) {  // lane per pair, wave-uniform weights
+    const long u = sh.u, d = sh.n - sh.u;
+    const int blocks_same = (int)((sh.B * (u * u + d * d) + 255) / 256);
+    const int blocks_diff = (int)((sh.B * 2 * u * d + 255) / 256);
+    hipLaunchKernelGGL(k_dpe4_pairs_value<T>, dim3(blocks_same + blocks_diff), dim3(256), 0, s, r, Wp, bp, wm, sh,
+                       blocks_same);
+    return;
+  }
   const unsigned grid = (unsigned)(sh.B * ((sh.n + 7) / 8));
   const int NS = 2 * sh.L - 1;
   constexpr bool WLDS = sizeof(T) == 4;
-  if (sh.C > 1) {
-    const size_t smem = ((size_t)8 * 5 * EMB + (WLDS ? (size_t)NS * 2 * (EMB * EMB + EMB) : 0)) * sizeof(T);
-    hipLaunchKernelGGL((k_dpe4_pairs<true, WLDS, T>), dim3(grid), dim3(256), smem, s, r, Wp, bp, wm, sh);
-  } else {
-    const size_t smem = ((size_t)8 * EMB + (WLDS ? (size_t)NS * 2 * (EMB * EMB + EMB) : 0)) * sizeof(T);
-    hipLaunchKernelGGL((k_dpe4_pairs<false, WLDS, T>), dim3(grid), dim3(256), smem, s, r, Wp, bp, wm, sh);
-  }
+  const size_t smem = ((size_t)8 * 5 * EMB + (WLDS ? (size_t)NS * 2 * (EMB * EMB + EMB) : 0)) * sizeof(T);
+  hipLaunchKernelGGL((k_dpe4_pairs<true, WLDS, T>), dim3(grid), dim3(256), smem, s, r, Wp, bp, wm, sh);
 }
 
 template <typename T>
@@ -410,15 +511,30 @@ void launch_dpe4_tanh(T* hm, const T* bias, const Shape& sh, hipStream_t s) {
   hipLaunchKernelGGL(k_dpe4_tanh<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, hm, bias, sh);
 }
 
+template <typename T, int RB>
+static void conv_launch(const T* wm, const T* hm, T* X, int Kx, int col0, const Shape& sh, hipStream_t s) {
+  const int nRB = (sh.Cp + RB - 1) / RB;
+  const size_t smem = ((size_t)sh.n * RB * EMB + (size_t)sh.n * EMB) * sizeof(T);
+  hipLaunchKernelGGL((k_dpe4_conv<true, RB, T>), dim3((unsigned)(sh.B * nRB)), dim3(256), smem, s, wm, hm, X, Kx,
+                     col0, sh);
+}
+
 template <typename T>
 void launch_dpe4_conv(const T* wm, const T* hm, T* X, int Kx, int col0, const Shape& sh, hipStream_t s) {
-  constexpr int RB = 16;
-  const long threads = (long)sh.B * sh.n * (sh.C > 1 ? (sh.Cp + RB - 1) / RB : 1) * EMB;
-  const unsigned grid = (unsigned)(((threads + 255) / 256 + 7) / 8 * 8);  // a multiple of 8 (XCD order)
-  if (sh.C > 1)
-    hipLaunchKernelGGL((k_dpe4_conv<true, RB, T>), dim3(grid), dim3(256), 0, s, wm, hm, X, Kx, col0, sh, threads);
+  if (sh.C == 1) {
+    const size_t smem = (size_t)2 * sh.n * EMB * sizeof(T);
+    hipLaunchKernelGGL((k_dpe4_conv<false, 1, T>), dim3((unsigned)sh.B), dim3(256), smem, s, wm, hm, X, Kx, col0,
+                       sh);
+    return;
+  }
+  // channel rows per block: the staged hm block (n x RB x 32) stays within 48 KB of LDS
+  const size_t row_bytes = (size_t)sh.n * EMB * sizeof(T);
+  if (row_bytes * 16 <= 48 * 1024)
+    conv_launch<T, 16>(wm, hm, X, Kx, col0, sh, s);
+  else if (row_bytes * 8 <= 48 * 1024)
+    conv_launch<T, 8>(wm, hm, X, Kx, col0, sh, s);
   else
-    hipLaunchKernelGGL((k_dpe4_conv<false, 1, T>), dim3(grid), dim3(256), 0, s, wm, hm, X, Kx, col0, sh, threads);
+    conv_launch<T, 4>(wm, hm, X, Kx, col0, sh, s);
 }
 
 #define DPE4_INST(T)                                                                                               \
