@@ -21,7 +21,8 @@ DW_FP32, DW_BF16X3, DW_BF16, DW_FP64 = 0, 1, 2, 3
 PRECISIONS = {"fp32": DW_FP32, "bf16x3": DW_BF16X3, "bf16": DW_BF16, "fp64": DW_FP64}
 FLAG_NONFINITE, FLAG_FP64_DET, FLAG_FORCED_ACCEPT = 1, 2, 4
 TIMER_GEMM_1EL_LAP, TIMER_DET_LAP, TIMER_FORWARD, TIMER_LOCAL_ENERGY = 0, 1, 2, 3
-PROPOSALS = {"normal": 0, "local": 1, "langevin": 2}
+PROPOSALS = {"normal": 0, "local": 1, "langevin": 2, "normal_one_el": 3, "local_one_el": 4}
+ONE_EL_PROPOSALS = ("normal_one_el", "local_one_el")
 
 # Exported symbols and their ctypes signatures (argtypes, restype).
 _vp, _i32, _f32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_float
@@ -286,7 +287,10 @@ class NativeModel:
 
     def set_proposal(self, name: str = "normal", r_min: float = 0.1, r_max: float = 1.0,
                      langevin_scale: float = 1.0) -> None:
-        """dw_model_set_proposal: 'normal', 'local' or 'langevin' (the proposal names of MCMCConfig)."""
+        """dw_model_set_proposal with a proposal name of MCMCConfig ('cauchy' is the normal kernel fed
+        Cauchy draws)."""
+        if name == "cauchy":
+            name = "normal"
         if name not in PROPOSALS:
             raise ValueError(f"unknown proposal {name!r}; the device sampler has {sorted(PROPOSALS)}")
         _check(self.lib.dw_model_set_proposal(self.handle, PROPOSALS[name], float(r_min), float(r_max),

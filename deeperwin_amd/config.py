@@ -164,6 +164,7 @@ class ModelConfigDeepErwin4(ModelConfigFermiNet):
 
 @dataclass
 class MCMCSimpleProposalConfig:
+    """'normal', 'cauchy' or 'normal_one_el' (configuration.py:952-953)."""
     name: str = "normal"
 
 
@@ -178,7 +179,8 @@ class MCMCLangevinProposalConfig:
 
 @dataclass
 class LocalStepsizeProposalConfig:
-    """Stepsize scaled by the distance to the closest nucleus (configuration.py:966-975)."""
+    """Stepsize scaled by the distance to the closest nucleus, all electrons ('local') or one per step
+    ('local_one_el') (configuration.py:966-975)."""
     name: str = "local"
     r_min: float = 0.1
     r_max: float = 1.0
@@ -205,9 +207,10 @@ class MCMCConfig:
     stepsize: float = 0.02
 
     def validate(self) -> None:
-        if self.proposal.name not in ("normal", "local", "langevin"):
-            raise NotImplementedError(f"proposal {self.proposal.name!r} is not on the native path "
-                                      "(the device sampler has 'normal', 'local' and 'langevin')")
+        names = ("normal", "cauchy", "normal_one_el", "local", "local_one_el", "langevin")
+        if self.proposal.name not in names:
+            raise NotImplementedError(f"proposal {self.proposal.name!r} is not a proposal of the reference "
+                                      f"sampler ({', '.join(names)})")
         if self.p_spin_swap > 0 or self.p_spin_flip > 0:
             raise NotImplementedError("spin swap / spin flip moves are not on the native path")
 

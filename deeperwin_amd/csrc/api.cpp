@@ -829,7 +829,10 @@ dw_status mcmc_run(dw_model* m, const McmcArgs& a, const FixedScratch* fx, hipSt
   const bool dbl = m->prec == GEMM_F64;
   const long B = a.n_walkers;
   const int cnt3 = (int)(B * m->n * 3);
-  const bool scaled = m->prop_kind != DW_PROPOSAL_NORMAL;
+  const int kind = m->prop_kind;
+  const bool scaled = kind == DW_PROPOSAL_LOCAL || kind == DW_PROPOSAL_LANGEVIN || kind == DW_PROPOSAL_LOCAL_ONE_EL;
+  const bool one_el = kind == DW_PROPOSAL_NORMAL_ONE_EL || kind == DW_PROPOSAL_LOCAL_ONE_EL;
+  const size_t draw_stride = one_el ? (size_t)B * 3 : (size_t)cnt3;  // floats of draws per step
   float *rprop, *lpprop, *logq = nullptr;
   int32_t* cnt;
   PassBufs<float> pf;
@@ -886,11 +889,15 @@ dw_status mcmc_run(dw_model* m, const McmcArgs& a, const FixedScratch* fx, hipSt
   if (!st && a.n_steps > 0) {
     launch_fill_i32(cnt, 0, a.n_steps, s);
     for (int k = 0; k < a.n_steps && !st; ++k) {
-      if (scaled)
-        launch_propose_scaled(m->prop_kind, a.r, a.normals + (size_t)k * cnt3, a.chain, a.R, a.Z, m->I, m->prop_rmin,
-                              m->prop_rmax, m->prop_scale, rprop, logq, (int)B, m->n, s);
+      const float* xi = a.normals + (size_t)k * draw_stride;
+      if (one_el)
+        launch_propose_one_el(kind, a.r, xi, a.chain, a.R, m->I, m->prop_rmin, m->prop_rmax, rprop, logq, (int)B,
+                              m->n, s);
+      else if (scaled)
+        launch_propose_scaled(kind, a.r, xi, a.chain, a.R, a.Z, m->I, m->prop_rmin, m->prop_rmax, m->prop_scale, rprop,
+                              logq, (int)B, m->n, s);
       else
-        launch_propose(a.r, a.normals + (size_t)k * cnt3, a.chain, rprop, cnt3, s);
+        launch_propose(a.r, xi, a.chain, rprop, cnt3, s);
       st = value_pass(rprop, lpprop);
       if (st) break;
       launch_accept(a.r, rprop, a.log_psi_sqr, lpprop, logq, a.uniforms + (size_t)k * B, a.walker_age, a.max_age,
@@ -1274,8 +1281,9 @@ dw_status dw_threefry_mcmc_draws(const uint32_t* keys_in, uint32_t* keys_out, in
 
 dw_status dw_model_set_proposal(dw_model* m, int32_t kind, float r_min, float r_max, float langevin_scale) {
   if (!m) return fail(DW_ERR_INVALID, "null model");
-  if (kind < DW_PROPOSAL_NORMAL || kind > DW_PROPOSAL_LANGEVIN) return fail(DW_ERR_INVALID, "unknown proposal kind");
-  if (kind != DW_PROPOSAL_NORMAL &&
+  if (kind < DW_PROPOSAL_NORMAL || kind > DW_PROPOSAL_LOCAL_ONE_EL)
+    return fail(DW_ERR_INVALID, "unknown proposal kind");
+  if ((kind == DW_PROPOSAL_LOCAL || kind == DW_PROPOSAL_LANGEVIN || kind == DW_PROPOSAL_LOCAL_ONE_EL) &&
       !(r_min > 0.f && r_max >= r_min && std::isfinite(r_max) && std::isfinite(langevin_scale)))
     return fail(DW_ERR_INVALID, "need 0 < r_min <= r_max (finite) and a finite langevin_scale");
   m->prop_kind = kind;
@@ -1292,9 +1300,15 @@ dw_status dw_mcmc_propose(dw_model* m, const float* r, int32_t n_walkers, const 
   if (n_walkers <= 0) return fail(DW_ERR_INVALID, "n_walkers must be positive");
   if (out_r == r) return fail(DW_ERR_INVALID, "out_r must not alias r");
   const hipStream_t s = (hipStream_t)stream;
-  if (m->prop_kind == DW_PROPOSAL_NORMAL) {
-    launch_propose(r, normals, chain, out_r, n_walkers * m->n * 3, s);
-    launch_fill_i32((int32_t*)out_log_q, 0, n_walkers, s);  // +0.0f; a kernel, so the call can be captured
+  const int kind = m->prop_kind;
+  if (kind == DW_PROPOSAL_NORMAL || kind == DW_PROPOSAL_NORMAL_ONE_EL || kind == DW_PROPOSAL_LOCAL_ONE_EL) {
+    if (kind == DW_PROPOSAL_NORMAL)
+      launch_propose(r, normals, chain, out_r, n_walkers * m->n * 3, s);
+    if (kind != DW_PROPOSAL_LOCAL_ONE_EL)
+      launch_fill_i32((int32_t*)out_log_q, 0, n_walkers, s);  // +0.0f; a kernel, so the call can be captured
+    if (kind != DW_PROPOSAL_NORMAL)
+      launch_propose_one_el(kind, r, normals, chain, R, m->I, m->prop_rmin, m->prop_rmax, out_r, out_log_q, n_walkers,
+                            m->n, s);
   } else {
     launch_propose_scaled(m->prop_kind, r, normals, chain, R, Z, m->I, m->prop_rmin, m->prop_rmax, m->prop_scale,
                           out_r, out_log_q, n_walkers, m->n, s);

@@ -133,6 +133,10 @@ void launch_propose(const float* r, const float* normals, const void* chain, flo
 void launch_propose_scaled(int kind, const float* r, const float* normals, const void* chain, const float* R,
                            const float* Z, int I, float r_min, float r_max, float lscale, float* r_prop, float* logq,
                            int B, int n, hipStream_t s);
+// One-electron moves (DW_PROPOSAL_NORMAL_ONE_EL / DW_PROPOSAL_LOCAL_ONE_EL, mcmc.py:183-192, 239-252): electron
+// step_nr mod n moves by xi [B, 3]; logq [B] written by the local form only.
+void launch_propose_one_el(int kind, const float* r, const float* xi, const void* chain, const float* R, int I,
+                           float r_min, float r_max, float* r_prop, float* logq, int B, int n, hipStream_t s);
 // logq (may be null for a symmetric proposal) is added to the exponent: accept iff exp(lp' - lp + logq) > u.
 void launch_accept(float* r, const float* r_prop, float* lp, const float* lp_prop, const float* logq,
                    const float* uniforms, int32_t* age, int max_age, int32_t* acc_count, uint32_t* flags, int B,

@@ -97,7 +97,61 @@ __global__ __launch_bounds__(64) void k_propose_scaled(const float* __restrict__
   if (i == 0) logq[b] = lq;
 }
 
+// One-electron moves (mcmc.py:183-192, 239-252): only electron k = step_nr mod n moves, by its 3 draws
+// xi [B, 3]; local form: s = stepsize * clip(distance of electron k to the closest nucleus), r'_k = r_k +
+// s xi and log q = 3 (log s - log s') + |r'_k - r_k|^2 (1/s^2 - 1/s'^2) / 2 of that electron alone (the
+// reference does not sum over electrons here).  One thread per (walker, electron): every thread copies its
+// electron, the thread of electron k moves it.
+template <bool LOCAL>
+__global__ __launch_bounds__(256) void k_propose_one_el(const float* __restrict__ r, const float* __restrict__ xi,
+                                                        const dw_chain_scalars* __restrict__ chain,
+                                                        const float* __restrict__ R, int I, float r_min, float r_max,
+                                                        float* __restrict__ r_prop, float* __restrict__ logq, int B,
+                                                        int n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)B * n) return;
+  const long b = t / n;
+  const int i = (int)(t - b * n);
+  const int k = (int)(((long)chain->step_nr % n + n) % n);
+  const float step = chain->stepsize;
+  const long o = t * 3;
+  float x[3] = {r[o], r[o + 1], r[o + 2]};
+  if (i != k) {
+    r_prop[o] = x[0];
+    r_prop[o + 1] = x[1];
+    r_prop[o + 2] = x[2];
+    return;
+  }
+  const float* d = xi + b * 3;
+  if (!LOCAL) {
+    for (int c = 0; c < 3; ++c) r_prop[o + c] = x[c] + d[c] * step;
+    return;
+  }
+  float s, g[3];
+  scale_and_drift<false>(x, R, nullptr, I, step, r_min, r_max, 0.f, s, g);
+  float xn[3];
+  for (int c = 0; c < 3; ++c) xn[c] = x[c] + d[c] * s;
+  float sn;
+  scale_and_drift<false>(xn, R, nullptr, I, step, r_min, r_max, 0.f, sn, g);
+  for (int c = 0; c < 3; ++c) r_prop[o + c] = xn[c];
+  float d2 = 0.f;
+  for (int c = 0; c < 3; ++c) d2 += (xn[c] - x[c]) * (xn[c] - x[c]);
+  logq[b] = 3.f * (logf(s) - logf(sn)) + 0.5f * d2 * (1.f / (s * s) - 1.f / (sn * sn));
+}
+
 }  // namespace
+
+void launch_propose_one_el(int kind, const float* r, const float* xi, const void* chain, const float* R, int I,
+                           float r_min, float r_max, float* r_prop, float* logq, int B, int n, hipStream_t s) {
+  const dw_chain_scalars* c = (const dw_chain_scalars*)chain;
+  const unsigned grid = (unsigned)(((long)B * n + 255) / 256);
+  if (kind == DW_PROPOSAL_LOCAL_ONE_EL)
+    hipLaunchKernelGGL(k_propose_one_el<true>, dim3(grid), dim3(256), 0, s, r, xi, c, R, I, r_min, r_max, r_prop, logq,
+                       B, n);
+  else
+    hipLaunchKernelGGL(k_propose_one_el<false>, dim3(grid), dim3(256), 0, s, r, xi, c, R, I, r_min, r_max, r_prop,
+                       logq, B, n);
+}
 
 void launch_propose_scaled(int kind, const float* r, const float* normals, const void* chain, const float* R,
                            const float* Z, int I, float r_min, float r_max, float lscale, float* r_prop, float* logq,

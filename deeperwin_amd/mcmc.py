@@ -2,10 +2,10 @@
 
 ``MetropolisHastingsMonteCarlo(mcmc_config).run_mcmc_steps(func, state, params, n_up, n_dn,
 fixed_params, n_steps)`` keeps the reference signature and returns a new ``MCMCState``.  The whole chain
-of ``n_steps`` steps runs on the device without host synchronisation: proposals (the config's 'normal',
-'local' or 'langevin' rule, mcmc.py:175-180, 218-236, 255-284), log|psi|^2 of the proposals,
-accept/reject, walker ages, the running acceptance rate and the stepsize schedule (stepsize, acc_rate and
-step_nr live in a 16-byte device record).
+of ``n_steps`` steps runs on the device without host synchronisation: proposals (the config's rule:
+'normal', 'cauchy', 'normal_one_el', 'local', 'local_one_el' or 'langevin', mcmc.py:175-284), log|psi|^2 of
+the proposals, accept/reject, walker ages, the running acceptance rate and the stepsize schedule
+(stepsize, acc_rate and step_nr live in a 16-byte device record).
 
 Random draws: the reference derives proposals and accept thresholds from per-walker threefry keys
 (mcmc.py:178, 360-361).  When ``state.rng_state`` holds such keys (a CUDA int32 tensor [B, 2] of uint32
@@ -113,14 +113,27 @@ class MetropolisHastingsMonteCarlo:
         self.config = mcmc_config
 
     def draw(self, state: MCMCState, n_steps: int):
-        """(normals, uniforms, next rng_state) for n_steps steps of the walkers in `state`."""
+        """(step draws, uniforms, next rng_state) for n_steps steps of the walkers in `state`: normals
+        [S, B, n, 3] ([S, B, 3] for the one-electron proposals, whose key draws normal(subkey, (3,))),
+        standard Cauchy variates for the 'cauchy' proposal."""
+        from ._native import ONE_EL_PROPOSALS
+
         B, n, _ = state.r.shape
+        name = self.config.proposal.name
+        m = 1 if name in ONE_EL_PROPOSALS else n
         g = state.rng_state
         if isinstance(g, torch.Tensor):  # per-walker threefry keys: the reference's own streams
             from . import threefry
 
-            return threefry.mcmc_draws(g, n, n_steps)
-        normals = torch.randn((n_steps, B, n, 3), generator=g, device="cuda", dtype=torch.float32)
+            if name == "cauchy":
+                raise NotImplementedError("JAX's Cauchy stream is not replayed; use a torch.Generator or draws=")
+            normals, uniforms, keys = threefry.mcmc_draws(g, m, n_steps)
+            return (normals.reshape(n_steps, B, 3) if m == 1 else normals), uniforms, keys
+        shape = (n_steps, B, 3) if m == 1 else (n_steps, B, n, 3)
+        if name == "cauchy":
+            normals = torch.empty(shape, device="cuda", dtype=torch.float32).cauchy_(generator=g)
+        else:
+            normals = torch.randn(shape, generator=g, device="cuda", dtype=torch.float32)
         uniforms = torch.rand((n_steps, B), generator=g, device="cuda", dtype=torch.float32)
         return normals, uniforms, g
 
@@ -144,8 +157,12 @@ class MetropolisHastingsMonteCarlo:
                 normals, uniforms, new.rng_state = self.draw(state, n_steps)
             normals = _as_device(normals)
             uniforms = _as_device(uniforms)
-            if normals.shape != (n_steps, B, n_up + n_dn, 3) or uniforms.shape != (n_steps, B):
-                raise ValueError("draws must have shapes [n_steps, B, n_el, 3] and [n_steps, B]")
+            from ._native import ONE_EL_PROPOSALS
+
+            step_shape = (n_steps, B, 3) if self.config.proposal.name in ONE_EL_PROPOSALS else (
+                n_steps, B, n_up + n_dn, 3)
+            if normals.shape != step_shape or uniforms.shape != (n_steps, B):
+                raise ValueError(f"draws must have shapes {list(step_shape)} and [n_steps, B]")
         else:
             normals = uniforms = None
         acc = torch.empty(max(n_steps, 1), device="cuda", dtype=torch.float32)

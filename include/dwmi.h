@@ -113,26 +113,33 @@ typedef struct {
 #define DW_PROPOSAL_NORMAL 0   /* r' = r + stepsize xi                                  (mcmc.py:175-180) */
 #define DW_PROPOSAL_LOCAL 1    /* per-electron stepsize from the closest nucleus        (mcmc.py:218-236) */
 #define DW_PROPOSAL_LANGEVIN 2 /* local stepsize plus a drift towards the nuclei        (mcmc.py:255-284) */
+#define DW_PROPOSAL_NORMAL_ONE_EL 3 /* only electron step_nr mod n_el moves, by stepsize xi (mcmc.py:183-192) */
+#define DW_PROPOSAL_LOCAL_ONE_EL 4  /* that electron with its local stepsize              (mcmc.py:239-252) */
 
 /* Select the proposal of dw_mcmc_steps / dw_mcmc_steps_noalloc / dw_mcmc_propose for this model (default
  * DW_PROPOSAL_NORMAL).  For LOCAL and LANGEVIN each electron moves with s = stepsize * clip(d, r_min, r_max),
  * d its distance to the closest nucleus; LANGEVIN adds g s^2 with g = -langevin_scale sum_I Z_I (r - R_I) /
  * |r - R_I|.  The accept test becomes exp(lp' - lp + log q) > u with log q the log ratio of the reverse to
- * the forward proposal density.  The reference's defaults: local r_min 0.1, r_max 1; langevin r_min 0.2,
- * r_max 2, langevin_scale 1 (configuration.py:956-975).  Needs 0 < r_min <= r_max.  Replaces the choice of
- * self.propose in MetropolisHastingsMonteCarlo.__init__ (mcmc.py:330-342) for these three names. */
+ * the forward proposal density.  The ONE_EL forms move only electron k = step_nr mod n_el of every walker
+ * (step_nr of the chain record before the step) and take 3 draws per walker and step: normals
+ * [n_steps, B, 3]; LOCAL_ONE_EL's log q is that electron's term alone, as in the reference.  The Cauchy
+ * proposal (mcmc.py:195-200) is DW_PROPOSAL_NORMAL fed Cauchy draws.  The reference's defaults: local r_min
+ * 0.1, r_max 1; langevin r_min 0.2, r_max 2, langevin_scale 1 (configuration.py:956-975).  Needs
+ * 0 < r_min <= r_max.  Replaces the choice of self.propose in MetropolisHastingsMonteCarlo.__init__
+ * (mcmc.py:330-342). */
 dw_status dw_model_set_proposal(dw_model* m, int32_t kind, float r_min, float r_max, float langevin_scale);
 
 /* One proposal of the selected rule without evaluating the wavefunction: out_r [B, n_el, 3] and
- * out_log_q [B] (0 for DW_PROPOSAL_NORMAL) from r [B, n_el, 3], normals [B, n_el, 3] and the chain's
- * stepsize.  Replaces the propose functions `_propose_normal`, `_propose_step_local_stepsize` and
- * `_propose_step_local_approximated_langevin` (mcmc.py:175-180, 218-236, 255-284) with supplied draws. */
+ * out_log_q [B] (0 for the symmetric rules) from r [B, n_el, 3], normals [B, n_el, 3] ([B, 3] for the ONE_EL
+ * rules) and the chain's stepsize (and step_nr).  Replaces the propose functions `_propose_normal`,
+ * `_propose_normal_one_el`, `_propose_step_local_stepsize`, `_propose_step_local_stepsize_one_el` and
+ * `_propose_step_local_approximated_langevin` (mcmc.py:175-192, 218-284) with supplied draws. */
 dw_status dw_mcmc_propose(dw_model* m, const float* r, int32_t n_walkers, const float* R, const float* Z,
                           const float* normals, const dw_chain_scalars* chain, float* out_r, float* out_log_q,
                           dw_stream stream);
 
 /* n_steps Metropolis-Hastings steps with the model's proposal rule (dw_model_set_proposal) and SUPPLIED
- * draws: normals [n_steps, B, n_el, 3], uniforms [n_steps, B].  r, log_psi_sqr and walker_age are updated
+ * draws: normals [n_steps, B, n_el, 3] ([n_steps, B, 3] for the ONE_EL rules), uniforms [n_steps, B].  r, log_psi_sqr and walker_age are updated
  * in place; chain (device dw_chain_scalars) carries stepsize / acc_rate / step_nr; out_acc [n_steps]
  * receives each step's acceptance fraction (may be NULL).  log_psi_sqr is re-evaluated on entry.
  * Replaces MetropolisHastingsMonteCarlo.run_mcmc_steps (mcmc.py:325-327, 345-406). */

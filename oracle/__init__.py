@@ -92,6 +92,8 @@ PINNING = {
         "test_oracle_proposals.py::test_stationary_distribution_hydrogen_1s (local and Langevin chains; fails "
         "without log q)",
         "test_oracle_proposals.py::test_constant_scale_without_drift_is_symmetric",
+        "test_oracle_proposals.py::test_one_electron_proposals (one-electron forms, scipy densities)",
+        "test_oracle_proposals.py::test_one_electron_chain_samples_two_independent_electrons",
     ],
     "mcmc.log_q_ratio": [
         "test_oracle_proposals.py::test_log_q_is_reverse_over_forward_density (through propose)",

@@ -71,15 +71,32 @@ def log_q_ratio(kind: str, r, r_new, R, Z, stepsize, r_min=0.1, r_max=1.0, lange
     return np.sum(lq, axis=-1).astype(dtype)
 
 
-def propose(kind: str, r, R, Z, normals, stepsize, r_min=0.1, r_max=1.0, langevin_scale=1.0, dtype=np.float32):
+def propose(kind: str, r, R, Z, normals, stepsize, r_min=0.1, r_max=1.0, langevin_scale=1.0, dtype=np.float32,
+            step_nr: int = 0):
     """One proposal (r' [B, n, 3], log q [B]) of the reference's rules, in the reference's operation order:
 
-    * 'normal'   r' = r + stepsize xi                      (mcmc.py:175-180)
+    * 'normal'   r' = r + stepsize xi                      (mcmc.py:175-180; 'cauchy' is the same with
+                                                            Cauchy draws, 195-200)
     * 'local'    r' = r + s xi                             (mcmc.py:218-236)
     * 'langevin' r' = r + s xi + g s^2                     (mcmc.py:255-284)
+    * 'normal_one_el' / 'local_one_el': only electron k = step_nr mod n moves, by xi [B, 3] (times stepsize,
+      or its local s); the local form's log q is that electron's term alone (mcmc.py:183-192, 239-252)
     with s, g from scale_and_drift at r, and log q from log_q_ratio."""
     r = np.asarray(r, dtype)
     xi = np.asarray(normals, dtype)
+    if kind in ("normal_one_el", "local_one_el"):
+        k = step_nr % r.shape[-2]
+        r_new = r.copy()
+        if kind == "normal_one_el":
+            r_new[:, k] = (r[:, k] + xi * dtype(stepsize)).astype(dtype)
+            return r_new, np.zeros(r.shape[0], dtype)
+        s, _ = scale_and_drift(r[:, k:k + 1], R, Z, stepsize, r_min, r_max, 0.0, dtype)
+        r_new[:, k] = (r[:, k] + xi * s).astype(dtype)
+        s_new, _ = scale_and_drift(r_new[:, k:k + 1], R, Z, stepsize, r_min, r_max, 0.0, dtype)
+        s, s_new = s[:, 0], s_new[:, 0]
+        dist_sqr = np.sum((r_new[:, k] - r[:, k]) ** 2, axis=-1)
+        lq = dtype(3) * (np.log(s) - np.log(s_new)) + dtype(0.5) * dist_sqr * (dtype(1) / s ** 2 - dtype(1) / s_new ** 2)
+        return r_new, lq.astype(dtype)
     if kind == "normal":
         r_new = r + xi * dtype(stepsize)
     elif kind in ("local", "langevin"):
@@ -98,7 +115,7 @@ def mcmc_steps(logpsi_fn: Callable[[np.ndarray], np.ndarray], state: ChainState,
                lo: float = 1e-2, hi: float = 1.0, dtype=np.float32, proposal=None):
     """Run S = len(normals) steps.  ``logpsi_fn(r[B, n, 3]) -> log_psi_sqr[B]``.
 
-    ``proposal(r, xi, stepsize) -> (r', log q)`` (default: the Gaussian rule, log q = 0).
+    ``proposal(r, xi, stepsize, step_nr) -> (r', log q)`` (default: the Gaussian rule, log q = 0).
     Per step (mcmc.py:345-379): r', log q from the proposal; p = exp(lp' - lp + log q); accept = (p > u) or
     (old age >= max_age); age <- 0 if accepted else age + 1; r, lp selected; acc = mean(accept);
     step_nr += 1; acc_rate <- 0.9 acc_rate_old + 0.1 acc; when step_nr % interval == 0 the stepsize is
@@ -114,7 +131,7 @@ def mcmc_steps(logpsi_fn: Callable[[np.ndarray], np.ndarray], state: ChainState,
         if proposal is None:
             r_new, log_q = (r + normals[s].astype(dtype) * stepsize).astype(dtype), dtype(0)
         else:
-            r_new, log_q = proposal(r, normals[s], stepsize)
+            r_new, log_q = proposal(r, normals[s], stepsize, step_nr)
         lp_new = logpsi_fn(r_new).astype(dtype)
         with np.errstate(over="ignore"):
             p_accept = np.exp((lp_new - lp + log_q).astype(dtype))

@@ -211,3 +211,131 @@ def test_dropin_and_noalloc_with_proposal(cuda, kind):
                      torch.zeros(B, dtype=torch.int32, device="cuda"), normals, uniforms, make_chain(0.1), 3, 4,
                      0.5, 1e-2, 1.0)
     assert not np.array_equal(r.cpu().numpy(), outs[0][0])
+
+
+@pytest.mark.parametrize("name", ["n2", "big60"])
+@pytest.mark.parametrize("kind", ["normal_one_el", "local_one_el"])
+def test_one_electron_propose_matches_oracle(cuda, name, kind):
+    """Only electron step_nr mod n moves (mcmc.py:183-192, 239-252): the other electrons are copied bit for
+    bit, the moved one and log q match the FP64 oracle (log q of the device's own move)."""
+    from deeperwin_amd.mcmc import make_chain
+
+    shape, params, R, Z, mapping = system(name, n_det=2, seed=1)
+    B, step_nr = 256, 17
+    k = step_nr % shape.n_el
+    m = _model(shape, params, "fp32", "local")
+    m.set_proposal(kind, 0.1, 1.0)
+    r = walkers(R, mapping, B, seed=7)
+    xi = np.random.default_rng(8).standard_normal((B, 3)).astype(np.float32)
+    out_r = torch.empty((B, shape.n_el, 3), device="cuda")
+    out_q = torch.full((B,), float("nan"), device="cuda")
+    m.propose(_dev(r), _dev(R), _dev(Z), _dev(xi), make_chain(0.3, 0.0, step_nr), out_r, out_q)
+    torch.cuda.synchronize()
+    r_dev = out_r.cpu().numpy()
+    np.testing.assert_array_equal(np.delete(r_dev, k, axis=1), np.delete(r, k, axis=1))
+    step = float(np.float32(0.3))
+    r_ref, _ = om.propose(kind, r.astype(np.float64), R, Z, xi.astype(np.float64), step, 0.1, 1.0,
+                          dtype=np.float64, step_nr=step_nr)
+    np.testing.assert_allclose(r_dev[:, k], r_ref[:, k], rtol=0, atol=2e-6 * (1 + np.abs(r[:, k]).max()))
+    q_dev = out_q.cpu().numpy().astype(np.float64)
+    if kind == "normal_one_el":
+        assert np.all(q_dev == 0)
+    else:
+        rd = r_dev.astype(np.float64)
+        s0, _ = om.scale_and_drift(r.astype(np.float64)[:, k:k + 1], R, Z, step, 0.1, 1.0, 0.0, np.float64)
+        s1, _ = om.scale_and_drift(rd[:, k:k + 1], R, Z, step, 0.1, 1.0, 0.0, np.float64)
+        d2 = np.sum((rd[:, k] - r[:, k]) ** 2, axis=-1)
+        q_ref = 3 * (np.log(s0[:, 0]) - np.log(s1[:, 0])) + 0.5 * d2 * (1 / s0[:, 0] ** 2 - 1 / s1[:, 0] ** 2)
+        tol = 1e-5 * (1 + 3 * np.abs(np.log(s0[:, 0])) + 3 * np.abs(np.log(s1[:, 0])) + 0.5 * d2 / s0[:, 0] ** 2 +
+                      0.5 * d2 / s1[:, 0] ** 2)
+        assert np.all(np.abs(q_dev - q_ref) <= tol), np.abs(q_dev - q_ref).max()
+        assert np.abs(q_ref).max() > 1e-3
+
+
+def test_one_electron_chain_lockstep_n2(cuda):
+    """local_one_el MCMC of the FP64 form in lockstep with the FP64 oracle: step k moves electron k mod n of
+    every walker; decisions agree outside the float32 rounding margin of the threshold."""
+    from deeperwin_amd.mcmc import make_chain, read_chain
+
+    shape, params, R, Z, mapping = system("n2", seed=1)
+    B, S = 128, 6
+    rng = np.random.default_rng(41)
+    normals = rng.standard_normal((S, B, 3)).astype(np.float32)
+    uniforms = rng.uniform(size=(S, B)).astype(np.float32)
+    m = _model(shape, params, "fp64", "local")
+    m.set_proposal("local_one_el", 0.1, 1.0)
+    r = _dev(walkers(R, mapping, B, seed=42))
+    lp = torch.full((B,), -1000.0, device="cuda")
+    age = torch.zeros(B, dtype=torch.int32, device="cuda")
+    chain = make_chain(0.5)
+    Rd, Zd = _dev(R), _dev(Z)
+    moved_total = 0
+    for k in range(S):
+        r_before = r.cpu().numpy()
+        step, _, step_nr = read_chain(chain)
+        assert step_nr == k
+        m.mcmc_steps(r, Rd, Zd, lp, age, _dev(normals[k:k + 1]), _dev(uniforms[k:k + 1]), chain, 100, 1000, 0.5,
+                     1e-2, 1.0)
+        torch.cuda.synchronize()
+        r_after = r.cpu().numpy()
+        dev_accept = age.cpu().numpy() == 0
+        e = k % shape.n_el
+        np.testing.assert_array_equal(np.delete(r_after, e, axis=1), np.delete(r_before, e, axis=1))
+        rp, lq = om.propose("local_one_el", r_before.astype(np.float64), R, Z, normals[k].astype(np.float64), step,
+                            0.1, 1.0, dtype=np.float64, step_nr=k)
+        lp0 = owf.log_psi_sqr(params, shape.n_up, shape.n_dn, torch.tensor(r_before.astype(np.float64)), R)[1].numpy()
+        lp1 = owf.log_psi_sqr(params, shape.n_up, shape.n_dn, torch.tensor(rp), R)[1].numpy()
+        p = np.exp(np.minimum(lp1 - lp0 + lq, 80.0))
+        ref_accept = p > uniforms[k]
+        margin = 1e-6 + p * ((np.abs(lp0) + np.abs(lp1)) * U32 + 1e-5 * (1 + np.abs(lq)))
+        near = np.abs(p - uniforms[k]) <= margin
+        assert np.all((dev_accept == ref_accept) | near)
+        moved_total += int(dev_accept.sum())
+    assert moved_total > 0
+
+
+def test_cauchy_and_one_electron_dropin(cuda):
+    """The drop-in sampler with 'cauchy' (the normal kernel on Cauchy draws) and 'normal_one_el' equals the
+    ABI on the same draws; one-electron draws from the walkers' threefry keys are normal(subkey, (3,)) of the
+    oracle's key chain, bit for bit on the uniforms."""
+    from deeperwin_amd import threefry
+    from deeperwin_amd.config import (MCMCConfig, MCMCSimpleProposalConfig, ModelConfigFermiNet,
+                                      OrbitalsConfigFermiNet, PhysicalConfig)
+    from deeperwin_amd.mcmc import MCMCState, MetropolisHastingsMonteCarlo, make_chain
+    from deeperwin_amd.physics import geometry_lih
+    from deeperwin_amd.wavefunction import build_log_psi_squared
+    from oracle import threefry as otf
+
+    R, Z = geometry_lih()
+    phys = PhysicalConfig(R=np.asarray(R, np.float32).astype(np.float64).tolist(), Z=list(map(int, Z)))
+    log_psi_sqr, _, _, params, fixed = build_log_psi_squared(
+        ModelConfigFermiNet(orbitals=OrbitalsConfigFermiNet(n_determinants=4)), phys, rng_seed=0)
+    n_up, n_dn = phys.n_up, phys.n_dn
+    n = n_up + n_dn
+    B, S = 64, 6
+    rng = np.random.default_rng(13)
+    for name, shape in (("cauchy", (S, B, n, 3)), ("normal_one_el", (S, B, 3))):
+        state = MCMCState.initialize_around_nuclei(B, phys, rng=5, stepsize=0.05)
+        draws = (_dev(rng.standard_cauchy(shape) if name == "cauchy" else rng.standard_normal(shape)),
+                 _dev(rng.uniform(size=(S, B))))
+        mc = MetropolisHastingsMonteCarlo(MCMCConfig(max_age=3, stepsize_update_interval=4,
+                                                     proposal=MCMCSimpleProposalConfig(name)))
+        new = mc.run_mcmc_steps(log_psi_sqr, state, params, n_up, n_dn, fixed, S, draws=draws)
+        model = log_psi_sqr.model_for(params, n_up, n_dn)
+        model.set_proposal(name)
+        r = state.r.clone()
+        lp = torch.full((B,), -1000.0, device="cuda")
+        age = torch.zeros(B, dtype=torch.int32, device="cuda")
+        model.mcmc_steps(r, state.R, state.Z, lp, age, draws[0], draws[1], make_chain(0.05), 3, 4, 0.5, 1e-2, 1.0)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(new.r.cpu().numpy(), r.cpu().numpy())
+        np.testing.assert_array_equal(new.walker_age.cpu().numpy(), age.cpu().numpy())
+    # threefry one-electron draws
+    key = threefry.PRNGKey(11)
+    state = MCMCState.initialize_around_nuclei(B, phys, rng=key, stepsize=0.05)
+    mc = MetropolisHastingsMonteCarlo(MCMCConfig(proposal=MCMCSimpleProposalConfig("normal_one_el")))
+    xi, u, keys = mc.draw(state, 3)
+    xi_ref, u_ref, keys_ref = otf.mcmc_draws(threefry.uint32(state.rng_state), 1, 3)
+    np.testing.assert_array_equal(u.cpu().numpy(), u_ref)
+    np.testing.assert_array_equal(threefry.uint32(keys), keys_ref)
+    np.testing.assert_allclose(xi.cpu().numpy(), xi_ref.reshape(3, B, 3), rtol=4e-6, atol=1e-6)

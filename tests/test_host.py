@@ -128,8 +128,11 @@ def test_config_validation():
     with pytest.raises(NotImplementedError):
         c.validate()
     m = MCMCConfig()
-    m.proposal.name = "cauchy"
+    m.proposal.name = "gaussian_mixture"
     with pytest.raises(NotImplementedError):
+        m.validate()
+    for name in ("cauchy", "normal_one_el"):
+        m.proposal.name = name
         m.validate()
     for prop in (LocalStepsizeProposalConfig(), MCMCLangevinProposalConfig()):
         MCMCConfig(proposal=prop).validate()

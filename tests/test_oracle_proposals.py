@@ -117,7 +117,7 @@ def test_stationary_distribution_hydrogen_1s(kind):
         normals = rng.standard_normal((S, B, 1, 3))
         uniforms = rng.uniform(size=(S, B))
 
-        def prop(r, xi, step):
+        def prop(r, xi, step, step_nr):
             r_new, lq = om.propose(kind, r, R, Z, xi, step, dtype=np.float64, **kw)
             return r_new, 0.0 * lq if drop_log_q else lq
 
@@ -154,3 +154,47 @@ def test_planted_mistakes_fail_the_density_check(mistake, monkeypatch):
     r_new, lq = om.propose(kind, r, R, Z, xi, 0.4, dtype=np.float64, **kw)
     want = _density_log_q(kind, r, r_new, R, Z, 0.4, kw["r_min"], kw["r_max"], kw["langevin_scale"])
     assert np.abs(lq - want).max() > 1e-3
+
+
+def test_one_electron_proposals():
+    """normal_one_el / local_one_el (mcmc.py:183-192, 239-252): only electron step_nr mod n moves; the local
+    form's log q is the reverse/forward density ratio of that electron alone (scipy.stats.norm)."""
+    r, R, Z, xi_all = _system(5, B=16, n=4)
+    xi = xi_all[:, 0]
+    for step_nr in (0, 5):
+        k = step_nr % 4
+        r1, lq1 = om.propose("normal_one_el", r, R, Z, xi, 0.3, dtype=np.float64, step_nr=step_nr)
+        assert np.all(lq1 == 0)
+        np.testing.assert_array_equal(np.delete(r1, k, axis=1), np.delete(r, k, axis=1))
+        np.testing.assert_allclose(r1[:, k], r[:, k] + 0.3 * xi, rtol=1e-12)
+        r2, lq2 = om.propose("local_one_el", r, R, Z, xi, 0.3, r_min=0.1, r_max=1.0, dtype=np.float64,
+                             step_nr=step_nr)
+        np.testing.assert_array_equal(np.delete(r2, k, axis=1), np.delete(r, k, axis=1))
+        for b in range(r.shape[0]):
+            s, _ = _scale_drift_loop(r[b, k], R, Z, 0.3, 0.1, 1.0, 0.0)
+            s1, _ = _scale_drift_loop(r2[b, k], R, Z, 0.3, 0.1, 1.0, 0.0)
+            np.testing.assert_allclose(r2[b, k], r[b, k] + s * xi[b], rtol=1e-12)
+            want = norm.logpdf(r[b, k], r2[b, k], s1).sum() - norm.logpdf(r2[b, k], r[b, k], s).sum()
+            assert abs(lq2[b] - want) <= 1e-9 * max(1.0, abs(want))
+
+
+def test_one_electron_chain_samples_two_independent_electrons():
+    """A chain of local_one_el moves (electron step_nr mod 2 each step) on |psi|^2 = exp(-2|r_1| - 2|r_2|)
+    samples <|r_i|> = 3/2 for both electrons."""
+    f = lambda r: -2.0 * np.linalg.norm(r, axis=-1).sum(axis=-1)
+    R, Z = np.zeros((1, 3)), np.ones(1)
+    B, S = 4000, 600
+    rng = np.random.default_rng(3)
+    st = om.ChainState(r=rng.standard_normal((B, 2, 3)), log_psi_sqr=np.full(B, -1000.0),
+                       walker_age=np.zeros(B, np.int64), stepsize=0.7)
+    normals = rng.standard_normal((S, B, 3))
+    uniforms = rng.uniform(size=(S, B))
+
+    def prop(r, x, step, step_nr):
+        return om.propose("local_one_el", r, R, Z, x, step, r_min=0.1, r_max=1.0, dtype=np.float64, step_nr=step_nr)
+
+    out, _, _ = om.mcmc_steps(f, st, normals, uniforms, max_age=1000, update_interval=10 ** 9, dtype=np.float64,
+                              proposal=prop)
+    rr = np.linalg.norm(out.r, axis=-1)
+    for e in range(2):
+        assert abs(rr[:, e].mean() - 1.5) < 5 * np.sqrt(0.75 / B)
