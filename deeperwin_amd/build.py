@@ -18,7 +18,7 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libdwmi.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = ["gemm.hip", "wf_kernels.hip", "api.cpp", "observables.hip", "grad.hip", "threefry.hip", "proposals.hip",
-           "dpe4.hip", "noalloc.hip"]
+           "dpe4.hip", "dpe4_grad.hip", "noalloc.hip"]
 COMMON = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Xclang", "-target-feature", "-Xclang",
           "-packed-fp32-ops", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 

@@ -114,10 +114,10 @@ struct Weights {
   DevBuf W2e, b2e, Wo[2], Wo_hi[2], Wo_lo[2], WoT[2], kappa, wenv;
   // dpe4: pair-stream sets, el-ion stream, ion maps, ion table; h_map GEMM operands per iteration
   DevBuf Wp, bp, Wei, bei, Whim, bhim, table;
-  std::vector<DevBuf> WhmT, WhmT_hi, WhmT_lo, bhm;
+  std::vector<DevBuf> WhmT, WhmT_hi, WhmT_lo, bhm, WhmB;  // WhmB: h_map w [pad128(Kh)][32] (gradient pass)
   void release() {
     for (auto* v : {&Wmain, &Wmu, &bias, &Wmain_hi, &Wmain_lo, &Wmu_hi, &Wmu_lo, &WmainT, &WmuT, &WhmT, &WhmT_hi,
-                    &WhmT_lo, &bhm})
+                    &WhmT_lo, &bhm, &WhmB})
       for (auto& b : *v) b.release();
     for (DevBuf* b : {&W2e, &b2e, &Wo[0], &Wo[1], &Wo_hi[0], &Wo_hi[1], &Wo_lo[0], &Wo_lo[1], &WoT[0], &WoT[1], &kappa,
                       &wenv, &Wp, &bp, &Wei, &bei, &Whim, &bhim, &table})
@@ -311,7 +311,7 @@ dw_status prepare_dpe4(dw_model* m, Weights& W, bool dbl, bool split) {
   if ((st = up(W.Wp, wp)) || (st = up(W.bp, bpv)) || (st = up(W.Wei, wei)) || (st = up(W.bei, beiv)) ||
       (st = up(W.Whim, whim)) || (st = up(W.bhim, bhimv)) || (st = up(W.table, leafp("input/h_ion", "embeddings").data)))
     return st;
-  for (auto* v : {&W.WhmT, &W.WhmT_hi, &W.WhmT_lo, &W.bhm}) v->resize(L);
+  for (auto* v : {&W.WhmT, &W.WhmT_hi, &W.WhmT_lo, &W.bhm, &W.WhmB}) v->resize(L);
   for (int l = 0; l < L; ++l) {
     char buf[96];
     std::snprintf(buf, sizeof buf, "symm_features_%d/convolutional_features/h_map/linear_0", l);
@@ -320,7 +320,9 @@ dw_status prepare_dpe4(dw_model* m, Weights& W, bool dbl, bool split) {
     std::vector<float> wt((size_t)128 * Khp, 0.f);
     for (int f = 0; f < E; ++f)
       for (int k = 0; k < Kh; ++k) wt[(size_t)f * Khp + k] = w.data[(size_t)k * E + f];
-    if ((st = up(W.WhmT[l], wt)) || (st = up(W.bhm[l], leafp(buf, "b").data))) return st;
+    std::vector<float> wb((size_t)pad_to(Kh, 128) * E, 0.f);
+    std::copy(w.data.begin(), w.data.end(), wb.begin());
+    if ((st = up(W.WhmT[l], wt)) || (st = up(W.bhm[l], leafp(buf, "b").data)) || (st = up(W.WhmB[l], wb))) return st;
     if (split && (st = upload_split(W.WhmT_hi[l], W.WhmT_lo[l], wt))) return st;
   }
   return DW_OK;
@@ -684,6 +686,11 @@ struct GradBufs {
   T* X2[3] = {nullptr, nullptr, nullptr};
   T* DP[3] = {nullptr, nullptr, nullptr};
   T *part = nullptr, *cpart = nullptr, *epart = nullptr;
+  // dpe4: h_map rows per layer, their gradient, the dh product, pair-map gradients (natural pair order), pair and
+  // el-ion stream rows, the el-ion products summed into dhim (float)
+  T *HMg[4] = {}, *dhm = nullptr, *tmp = nullptr, *DWp[4] = {}, *X2p[4] = {}, *DZm[4] = {}, *DZh[4] = {};
+  T *Q[4] = {}, *DZQ[4] = {}, *PH[4] = {};
+  float* dhim = nullptr;
 };
 
 constexpr int kEnvWalkersPerChunk = 32;
@@ -703,10 +710,27 @@ size_t layout_grad(const dw_model* m, long B, char* base, GradBufs<T>* out) {
   g.DZS = cv.take<T>((size_t)B * m->W1);
   g.dMU = cv.take<T>((size_t)B * kmmax);
   g.MU = cv.take<T>((size_t)B * kmmax);
-  for (int l = 1; l < m->L; ++l) g.dX[l] = cv.take<T>((size_t)rows * m->Kx[l]);
-  for (int l = 0; l < m->L - 1; ++l) {
-    g.X2[l] = cv.take<T>((size_t)pairs * (l == 0 ? 4 : 32));
-    g.DP[l] = cv.take<T>((size_t)pairs * 32);
+  for (int l = m->dpe4 ? 0 : 1; l < m->L; ++l) g.dX[l] = cv.take<T>((size_t)rows * m->Kx[l]);
+  if (!m->dpe4)
+    for (int l = 0; l < m->L - 1; ++l) {
+      g.X2[l] = cv.take<T>((size_t)pairs * (l == 0 ? 4 : 32));
+      g.DP[l] = cv.take<T>((size_t)pairs * 32);
+    }
+  const long eirows = rows * m->I;
+  if (m->dpe4) {
+    for (int l = 0; l < m->L; ++l) {
+      g.HMg[l] = cv.take<T>((size_t)rows * 32);
+      g.DWp[l] = cv.take<T>((size_t)pairs * 32);
+      g.X2p[l] = cv.take<T>((size_t)pairs * 32);
+      g.DZm[l] = cv.take<T>((size_t)pairs * 32);
+      if (l < m->L - 1) g.DZh[l] = cv.take<T>((size_t)pairs * 32);
+      g.Q[l] = cv.take<T>((size_t)eirows * 32);
+      if (l < m->L - 1) g.DZQ[l] = cv.take<T>((size_t)eirows * 32);
+      g.PH[l] = cv.take<T>((size_t)eirows * 32);
+    }
+    g.dhm = cv.take<T>((size_t)rows * 32);
+    g.tmp = cv.take<T>((size_t)rows * m->W1);
+    g.dhim = cv.take<float>((size_t)m->L * m->I * 32);
   }
   size_t pmax = 0;
   for (int l = 0; l < m->L; ++l) {
@@ -716,8 +740,16 @@ size_t layout_grad(const dw_model* m, long B, char* base, GradBufs<T>* out) {
   }
   pmax = std::max(pmax, wgrad_part_elems(32, 32, pairs));
   pmax = std::max(pmax, wgrad_part_elems(m->W1, m->NO, B * std::max(m->u, m->dn)));
+  if (m->dpe4) {
+    for (int l = 0; l < m->L; ++l) pmax = std::max(pmax, wgrad_part_elems(m->Kh[l], 32, rows));
+    pmax = std::max(pmax, wgrad_part_elems(32, 32, eirows));
+  }
   g.part = cv.take<T>(pmax);
-  g.cpart = cv.take<T>((size_t)std::max(colsum_chunks(pairs), colsum_chunks(B)) * std::max(32, m->W1));
+  size_t cmax = (size_t)std::max(colsum_chunks(pairs), colsum_chunks(B)) * std::max(32, m->W1);
+  if (m->dpe4)
+    cmax = std::max({cmax, (size_t)colsum_chunks(rows) * m->I * 32, (size_t)colsum_chunks(eirows) * 32,
+                     (size_t)colsum_chunks(rows) * 32});
+  g.cpart = cv.take<T>(cmax);
   g.epart = cv.take<T>((size_t)((B + kEnvWalkersPerChunk - 1) / kEnvWalkersPerChunk) * 4 * m->I * m->NO);
   if (out) *out = g;
   return cv.off;
@@ -766,26 +798,92 @@ dw_status run_grad(dw_model* m, const float* r, long B, const float* R, const fl
              s);
     mm(g.dbf + r0 * m->NOk, m->NOk, W.WoT[sp].as<T>(), m->NOk, g.dH + r0 * m->W1, m->W1, nr, m->W1, m->NOk);
   }
+  // dpe4: hm_l of every layer (the value pass keeps only the last one)
+  const long wm_layer = B * (long)m->n * m->n * 32;
+  if (m->dpe4)
+    for (int l = 0; l < m->L; ++l) {
+      mm(pb.X[l], m->Kx[l], W.WhmT[l].as<T>(), m->Kt0[l], g.HMg[l], 32, rows, 32, m->Kt0[l]);
+      launch_dpe4_tanh<T>(g.HMg[l], W.bhm[l].as<T>(), sh, s);
+    }
   // 1-el layers, last to first
   launch_dz_last<T>(g.dH, pb.H, g.dz, sh, s);
   for (int l = m->L - 1; l >= 0; --l) {
     char key[64];
     std::snprintf(key, sizeof key, "h_el_%d/linear_0", l);
-    const int Kh = m->Kh[l], Ke = m->Ke[l], Kx = m->Kx[l], Km = m->Km[l];
+    const int Kh = m->Kh[l], Kx = m->Kx[l], Km = m->Km[l];
     float* gw = off(key, "w");
     launch_walker_sum<T>(g.dz, m->W1, m->W1, g.DZS, sh, s);
     launch_spin_mean<T>(pb.X[l], Kx, Kh, g.MU, Km, sh, s);
     wgrad<T>(pb.X[l], Kx, g.dz, m->W1, Kh, m->W1, rows, g.part, gw, m->W1, s);
     wgrad<T>(g.MU, Km, g.DZS, m->W1, 2 * Kh, m->W1, B, g.part, gw + (long)Kh * m->W1, m->W1, s);
-    wgrad<T>(pb.X[l] + Kh, Kx, g.dz, m->W1, 2 * Ke, m->W1, rows, g.part, gw + 3L * Kh * m->W1, m->W1, s);
+    wgrad<T>(pb.X[l] + m->Kt0[l], Kx, g.dz, m->W1, m->Kt[l], m->W1, rows, g.part, gw + 3L * Kh * m->W1, m->W1, s);
     launch_colsum<T>(g.DZS, m->W1, B, m->W1, g.cpart, off(key, "b"), s);
-    if (l == 0) break;
+    if (l == 0 && !m->dpe4) break;
     mm(g.dz, m->W1, W.WmainT[l].as<T>(), m->W1, g.dX[l], Kx, rows, Kx, m->W1);
+    if (m->dpe4) {  // e_ee_l = conv(w_l, hm_l), hm_l = tanh(h_l W_map + b): gradients of W_map, b, w_l and h_l
+      char hk[96];
+      std::snprintf(hk, sizeof hk, "symm_features_%d/convolutional_features/h_map/linear_0", l);
+      launch_dpe4_conv_bwd<T>(g.dX[l], Kx, m->Kt0[l], pb.WM + l * wm_layer, g.HMg[l], g.dhm, g.DWp[l], sh, s);
+      launch_dtanh<T>(g.dhm, g.HMg[l], rows * 32, s);
+      wgrad<T>(pb.X[l], Kx, g.dhm, 32, Kh, 32, rows, g.part, off(hk, "w"), 32, s);
+      launch_colsum<T>(g.dhm, 32, rows, 32, g.cpart, off(hk, "b"), s);
+      if (l > 0) {
+        mm(g.dhm, 32, W.WhmB[l].as<T>(), 32, g.tmp, Kh, rows, Kh, 32);
+        launch_add_cols<T>(g.dX[l], Kx, g.tmp, Kh, rows, Kh, s);
+      }
+    }
+    if (l == 0) break;
     mm(g.DZS, m->W1, W.WmuT[l].as<T>(), m->W1, g.dMU, Km, B, Km, m->W1);
     launch_dz<T>(g.dX[l], Kx, g.dMU, Km, pb.X[l], Kx, Kh, g.dz, sh, s);
   }
-  // 2-el stream
-  if (m->L > 1) {
+  if (m->dpe4) {
+    // pair streams: rows grouped by spin relation (same-spin pairs of every walker, then diff-spin pairs)
+    T* wml[4] = {};
+    for (int l = 0; l < m->L; ++l) wml[l] = pb.WM + l * wm_layer;
+    launch_dpe4_pairs_bwd<T>(r, W.Wp.as<T>(), W.bp.as<T>(), g.DWp, wml, g.X2p, g.DZm, g.DZh, sh, s);
+    const long P0 = B * ((long)m->u * m->u + (long)m->dn * m->dn), P1 = B * 2L * m->u * m->dn;
+    for (int l = 0; l < m->L; ++l) {
+      const int kin = l == 0 ? 1 : 32;
+      for (int sp = 0; sp < 2; ++sp) {
+        const long r0 = sp ? P0 : 0, nr = sp ? P1 : P0;
+        if (nr == 0) continue;
+        char key[96];
+        std::snprintf(key, sizeof key, "symm_features_%d/convolutional_features/%s/linear_0", l,
+                      sp ? "w_diff" : "w_same");
+        wgrad<T>(g.X2p[l] + r0 * 32, 32, g.DZm[l] + r0 * 32, 32, kin, 32, nr, g.part, off(key, "w"), 32, s);
+        launch_colsum<T>(g.DZm[l] + r0 * 32, 32, nr, 32, g.cpart, off(key, "b"), s);
+        if (l < m->L - 1) {
+          std::snprintf(key, sizeof key, "%s_%d/linear_0", sp ? "h_diff" : "h_same", l);
+          wgrad<T>(g.X2p[l] + r0 * 32, 32, g.DZh[l] + r0 * 32, 32, kin, 32, nr, g.part, off(key, "w"), 32, s);
+          launch_colsum<T>(g.DZh[l] + r0 * 32, 32, nr, 32, g.cpart, off(key, "b"), s);
+        }
+      }
+    }
+    // el-ion stream and the ion maps
+    int ldx[4] = {}, col[4] = {};
+    for (int l = 0; l < m->L; ++l) {
+      ldx[l] = m->Kx[l];
+      col[l] = m->Kt0[l] + 32;
+    }
+    launch_dpe4_elion_bwd<T>(r, R, W.Wei.as<T>(), W.bei.as<T>(), pb.HIM, g.dX, ldx, col, g.Q, g.DZQ, g.PH, sh, s);
+    const long eirows = rows * m->I;
+    float* gW[4] = {};
+    float* gB[4] = {};
+    for (int l = 0; l < m->L; ++l) {
+      char key[96];
+      if (l < m->L - 1) {
+        std::snprintf(key, sizeof key, "h_el_ion_%d/linear_0", l);
+        wgrad<T>(g.Q[l], 32, g.DZQ[l], 32, l == 0 ? 4 : 32, 32, eirows, g.part, off(key, "w"), 32, s);
+        launch_colsum<T>(g.DZQ[l], 32, eirows, 32, g.cpart, off(key, "b"), s);
+      }
+      launch_colsum<T>(g.PH[l], m->I * 32, rows, m->I * 32, g.cpart, g.dhim + (long)l * m->I * 32, s);
+      std::snprintf(key, sizeof key, "symm_features_%d/convolutional_features/h_ion_map/linear_0", l);
+      gW[l] = off(key, "w");
+      gB[l] = off(key, "b");
+    }
+    launch_dpe4_him_bwd<T>(g.dhim, pb.HIM, W.table.as<T>(), W.Whim.as<T>(), Z, m->z_min, m->types, m->L, m->I, gW,
+                           gB, off("input/h_ion", "embeddings"), s);
+  } else if (m->L > 1) {  // 2-el stream
     launch_twoel_bwd<T>(r, g.dX, m->Kx.data(), m->Kh.data(), W.W2e.as<T>(), W.b2e.as<T>(), g.X2, g.DP, sh, s);
     for (int l = 0; l < m->L - 1; ++l) {
       char key[64];
@@ -1244,7 +1342,6 @@ dw_status dw_param_grad(dw_model* m, const float* r, int32_t n_walkers, const fl
   if (!m || !r || !R || !Z || !walker_weights || !out_grad) return fail(DW_ERR_INVALID, "null argument");
   if (n_walkers <= 0) return fail(DW_ERR_INVALID, "n_walkers must be positive");
   if (m->n > 52) return fail(DW_ERR_INVALID, "dw_param_grad supports up to 52 electrons (the FP64 inverse in LDS)");
-  if (m->dpe4) return fail(DW_ERR_INVALID, "dw_param_grad covers the FermiNet embedding only, not dpe4");
   hipStream_t s = (hipStream_t)stream;
   dw_status st = prepare(m, false, s);
   if (st) return st;

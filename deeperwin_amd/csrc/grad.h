@@ -59,6 +59,37 @@ template <typename T>
 void launch_twoel_bwd(const float* r, T* const* dX, const int* Kx, const int* Kh, const T* W2e, const T* b2e,
                       T* const* X2, T* const* DP, const Shape& sh, hipStream_t s);
 
+// ---- dpe4 embedding backward (dpe4_grad.hip) ----
+// dhm[b n + j][f] = sum_i de[b n + i][f] wm[b][i][j][f], dw[b][i][j][f] = de[b n + i][f] hm[b n + j][f], with de
+// the columns col0 .. col0 + 31 of dX (row pitch ldx).
+template <typename T>
+void launch_dpe4_conv_bwd(const T* dX, int ldx, int col0, const T* wm, const T* hm, T* dhm, T* dw, const Shape& sh,
+                          hipStream_t s);
+// d[t] *= 1 - y[t]^2
+template <typename T>
+void launch_dtanh(T* d, const T* y, long count, hipStream_t s);
+// dst[r, c] += src[r, c] for c < ncols
+template <typename T>
+void launch_add_cols(T* dst, int ldd, const T* src, int lds, long rows, int ncols, hipStream_t s);
+// Pair streams backward: per pair, from dw_l (natural [b][i][j][32] order) and the maps wm_l of the value pass,
+// the stream inputs X2[l], map pre-activation gradients DZm[l] and stream-layer gradients DZh[l] (l < L-1), in
+// rows grouped by spin relation (same-spin pairs of every walker, then the diff-spin pairs).
+template <typename T>
+void launch_dpe4_pairs_bwd(const float* r, const T* Wp, const T* bp, T* const* dw, T* const* wm, T* const* X2,
+                           T* const* DZm, T* const* DZh, const Shape& sh, hipStream_t s);
+// El-ion stream backward per (electron row, ion): de_ei_l from dX[l] columns col[l] ..; writes the stream inputs
+// Q[l], stream-layer gradients DZQ[l] (l < L-1) and PH[l] = de_ei_l * q_l (to be summed over rows into dhim).
+template <typename T>
+void launch_dpe4_elion_bwd(const float* r, const float* R, const T* Wei, const T* bei, const T* him,
+                           T* const* dX, const int* ldx, const int* col, T* const* Q, T* const* DZQ, T* const* PH,
+                           const Shape& sh, hipStream_t s);
+// Ion-map and ion-table gradients from dhim [L][I][32] (float); gW[l] / gB[l] / gTable are the leaves in the
+// packed gradient.  One workgroup.
+template <typename T>
+void launch_dpe4_him_bwd(const float* dhim, const T* him, const T* table, const T* Whim, const float* Z, int z_min,
+                         int n_types, int L, int I, float* const* gW, float* const* gB, float* gTable,
+                         hipStream_t s);
+
 // out[t] = (float) sum_c part[c * count + t], chunks in order.
 template <typename T>
 void launch_sum_parts(const T* part, int chunks, long count, float* out, hipStream_t s);

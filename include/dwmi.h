@@ -69,7 +69,8 @@ typedef struct {
  * e_ee[i] = sum_j w_same|diff(pair[i, j]) * tanh(h W_map + b)[j] and e_ei[i] = sum_I el_ion[i, I] *
  * tanh(h_ion W + b)[I] with h_ion = embeddings[Z_I - Z_min] (ferminet_embedding.py:52-188,
  * input_features.py:176-183); its leaves are listed at dw_model_set_param.  width2 (= emb_dim) must be 32.
- * The parameter gradient (dw_param_grad) covers FERMINET only. */
+ * dw_param_grad covers both embeddings (for DPE4 also the convolution maps, the pair and el-ion streams, the
+ * ion maps and the rows of the ion table that the molecule's nuclei use; unused rows get zero). */
 #define DW_EMBEDDING_FERMINET 0
 #define DW_EMBEDDING_DPE4 1
 

@@ -66,22 +66,23 @@ def total_energy(E_loc, clipping_state, **clip_cfg):
     return float(Ecm), (state, aux)
 
 
-def param_grad(params: Dict, n_up: int, n_dn: int, r, R, c) -> Dict:
-    """G = sum_b c_b d log_psi_sqr(r_b) / d theta in float64, as a dict shaped like `params`."""
+def param_grad(params: Dict, n_up: int, n_dn: int, r, R, c, Z=None) -> Dict:
+    """G = sum_b c_b d log_psi_sqr(r_b) / d theta in float64, as a dict shaped like `params` (Z: the nuclear
+    charges, read by the dpe4 model's ion embedding)."""
     leaves = {(p, k): torch.tensor(np.asarray(v, dtype=np.float64), requires_grad=True)
               for p, lv in params.items() for k, v in lv.items()}
     tparams = {}
     for (p, k), t in leaves.items():
         tparams.setdefault(p, {})[k] = t
     r = torch.as_tensor(np.asarray(r, dtype=np.float64))
-    lp = wf.log_psi_sqr(tparams, n_up, n_dn, r, R)[1]
+    lp = wf.log_psi_sqr(tparams, n_up, n_dn, r, R, Z)[1]
     (lp * torch.as_tensor(np.asarray(c, dtype=np.float64))).sum().backward()
     return {p: {k: leaves[(p, k)].grad.numpy() for k in lv} for p, lv in params.items()}
 
 
-def loss_gradient(params: Dict, n_up: int, n_dn: int, r, R, E_loc, clipping_state, **clip_cfg):
+def loss_gradient(params: Dict, n_up: int, n_dn: int, r, R, E_loc, clipping_state, Z=None, **clip_cfg):
     """value_and_grad of total_energy (loss_function.py:75-158): (loss, (state, aux)), grads, with the
     tangent jnp.dot(d log_psi_sqr, E_clip - E_mean_clip) / B."""
     loss, (state, aux) = total_energy(E_loc, clipping_state, **clip_cfg)
     c = (aux["E_loc_clipped"] - aux["E_mean_clipped"]) / len(aux["E_loc"])
-    return (loss, (state, aux)), param_grad(params, n_up, n_dn, r, R, c)
+    return (loss, (state, aux)), param_grad(params, n_up, n_dn, r, R, c, Z)

@@ -74,7 +74,7 @@ def test_parameter_ingestion_and_errors():
 
 def test_dpe4_parameter_ingestion():
     """The dpe4 model takes the reference's dpe4 leaves (conv maps, same/diff and el-ion streams, the ion
-    table) and refuses the FermiNet 2-el leaves; the parameter gradient refuses the dpe4 model on the host."""
+    table) and refuses the FermiNet 2-el leaves."""
     shape = ModelShape(14, 7, 2, 4, embedding="dpe4", z_min=7, n_ion_types=1)
     m = _native.NativeModel(shape)
     params = init_params(shape, 3)
@@ -90,9 +90,6 @@ def test_dpe4_parameter_ingestion():
     incomplete.pop("wf/input/h_ion")
     with pytest.raises(_native.DwError, match="not every parameter"):
         m2.set_params(incomplete)
-    fake = ctypes.c_void_p(256)
-    assert m.lib.dw_param_grad(m.handle, fake, 8, fake, fake, fake, fake, None, None) == 1
-    assert "dpe4" in m.lib.dw_last_error().decode()
 
 
 def test_missing_leaf_is_reported():

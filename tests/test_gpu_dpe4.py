@@ -136,3 +136,70 @@ def test_dpe4_dropin_and_mcmc_step(cuda):
         n_near += int(near.sum())
         assert np.all((dev_accept == ref_accept) | near)
     print(f"dpe4 drop-in chain: {S} steps of {B} walkers, {n_near} decisions within the rounding margin")
+
+
+def _compare_grads(grads, ref, rel):
+    worst = 0.0
+    for path, leaves in ref.items():
+        for leaf, g_ref in leaves.items():
+            g = grads[path][leaf].double().cpu().numpy()
+            assert g.shape == g_ref.shape, (path, leaf)
+            scale = np.linalg.norm(g_ref)
+            err = np.linalg.norm(g - g_ref) / max(scale, 1e-30)
+            if scale < 1e-12:  # e.g. rows of the ion table that no nucleus of the molecule uses
+                assert np.abs(g).max() <= 1e-9, (path, leaf)
+                continue
+            worst = max(worst, err)
+            assert err <= rel, (path, leaf, err)
+    return worst
+
+
+@pytest.mark.parametrize("name,B", [("lih", 4), ("n2", 3), ("odd17", 2)])
+def test_dpe4_param_grad_fp64_form_matches_oracle(cuda, name, B):
+    """dw_param_grad of the dpe4 model (every leaf: conv maps, same/diff and el-ion streams, ion maps, ion
+    table, 1-el layers, orbitals) against the oracle's autograd gradient, which tests/test_oracle_dpe4.py pins
+    to finite differences."""
+    from deeperwin_amd._native import NativeModel
+    from oracle import loss as ol
+
+    shape, params, R, Z, mapping = system_dpe4(name, n_det=2, seed=3)
+    print(f"dpe4 gradient {name}: n = {shape.n_el}, {B} walkers", flush=True)
+    r = walkers(R, mapping, B, seed=6)
+    c = np.random.default_rng(3).standard_normal(B)
+    m = NativeModel(shape, "fp64")
+    m.set_params(params)
+    lp = torch.empty(B, device="cuda")
+    g = m.param_grad(_dev(r), _dev(R), _dev(Z), _dev(c), out_lp=lp)
+    torch.cuda.synchronize()
+    ref = ol.param_grad(params, shape.n_up, shape.n_dn, r.astype(np.float64), R, c, Z)
+    worst = _compare_grads(m.unpack_grad(g, params), ref, 1e-6)
+    print(f"  worst relative leaf error {worst:.2e}", flush=True)
+
+
+def test_dpe4_value_and_grad_dropin_matches_oracle(cuda):
+    """build_value_and_grad_func over the dpe4 drop-in wavefunction: loss, clipping state and every gradient
+    leaf against the oracle's total_energy and autograd gradient."""
+    from deeperwin_amd.config import ClippingConfig, ModelConfigDeepErwin4, OrbitalsConfigFermiNet, PhysicalConfig
+    from deeperwin_amd.hamiltonian import build_local_energy
+    from deeperwin_amd.loss import build_value_and_grad_func, init_clipping_state
+    from deeperwin_amd.physics import geometry_lih
+    from deeperwin_amd.wavefunction import build_log_psi_squared
+    from oracle import loss as ol
+
+    Rg, Zg = geometry_lih()
+    phys = PhysicalConfig(R=np.asarray(Rg, np.float32).astype(np.float64).tolist(), Z=list(map(int, Zg)))
+    cfg = ModelConfigDeepErwin4(orbitals=OrbitalsConfigFermiNet(n_determinants=2))
+    log_psi_sqr, _, _, params, fixed = build_log_psi_squared(cfg, phys, rng_seed=1, precision="fp64")
+    get_e = build_local_energy(log_psi_sqr)
+    vg = build_value_and_grad_func(log_psi_sqr, get_e, ClippingConfig())
+    R, Z = np.asarray(phys.R), np.asarray(phys.Z)
+    r = walkers(R, [0, 0, 0, 1], 6, seed=9).astype(np.float32)
+    batch = (_dev(r), _dev(R), _dev(Z.astype(np.float32)), fixed)
+    (loss, (state, aux)), grads = vg(params, init_clipping_state(), (phys.n_up, phys.n_dn), batch)
+    e_ref = oh.local_energy(params, phys.n_up, phys.n_dn, r.astype(np.float64), R, Z).numpy()
+    (loss_ref, (state_ref, _)), g_ref = ol.loss_gradient(params, phys.n_up, phys.n_dn, r.astype(np.float64), R,
+                                                         e_ref, (0.0, 1e12), Z=Z)
+    assert float(loss) == pytest.approx(loss_ref, rel=1e-6)
+    np.testing.assert_allclose(state.cpu().numpy(), state_ref, rtol=1e-5)
+    worst = _compare_grads(grads, g_ref, 1e-5)
+    print(f"dpe4 value_and_grad drop-in: loss {float(loss):.6f}, worst relative leaf error {worst:.2e}", flush=True)

@@ -5,7 +5,8 @@ ferminet_embedding.py:97-188, 207-267), CPU only:
   split and block reshapes followed literally, Leibniz determinants) on tiny systems with non-zero biases;
 * planted mistakes in the oracle (same/diff maps swapped, the ion table read without the Z_min offset,
   the self pair dropped from the convolution) fail that comparison;
-* the oracle's kinetic energy of the dpe4 model equals central finite differences of its log|psi|^2.
+* the oracle's kinetic energy of the dpe4 model equals central finite differences of its log|psi|^2;
+* its parameter gradient (autograd) equals central finite differences on entries of every kind of dpe4 leaf.
 """
 import numpy as np
 import pytest
@@ -14,6 +15,7 @@ import torch
 import bruteforce as bf
 from deeperwin_amd.params import ModelShape, init_params
 from oracle import hamiltonian as oh
+from oracle import loss as ol
 from oracle import wavefunction as owf
 
 
@@ -108,3 +110,33 @@ def test_dpe4_kinetic_energy_finite_differences():
             g2 += ((fp - fm) / (2 * h)) ** 2
         ek_fd = -0.5 * (0.25 * g2 + 0.5 * lap)
         assert abs(ek_fd - ekin[b]) < 2e-5 * max(1.0, abs(ekin[b]))
+
+
+def test_dpe4_param_grad_finite_differences():
+    shape, p, R, Z, r = _system(3, 2, seed=13)
+    c = np.array([0.7, -1.3, 0.4])
+    g = ol.param_grad(p, 2, 1, r, R, c, Z)
+
+    def loss(params):
+        lp = owf.log_psi_sqr(params, 2, 1, torch.tensor(r), R, Z)[1].numpy()
+        return float(np.dot(c, lp))
+
+    h = 1e-6
+    rng = np.random.default_rng(0)
+    checked = 0
+    for path, leaves in p.items():
+        if "envelope_orbitals" in path:
+            continue
+        for leaf, v in leaves.items():
+            flat = v.reshape(-1)
+            for idx in rng.choice(flat.size, size=min(2, flat.size), replace=False):
+                q = {pp: {kk: vv.copy() for kk, vv in lv.items()} for pp, lv in p.items()}
+                q[path][leaf].reshape(-1)[idx] += h
+                fp = loss(q)
+                q[path][leaf].reshape(-1)[idx] -= 2 * h
+                fm = loss(q)
+                fd = (fp - fm) / (2 * h)
+                an = g[path][leaf].reshape(-1)[idx]
+                assert abs(fd - an) <= 1e-5 * max(1.0, abs(an)), (path, leaf, idx, fd, an)
+                checked += 1
+    assert checked > 40
