@@ -3,11 +3,11 @@
 //   C[M, N] = epi(A[M, K] * Bt[N, K]^T)
 //
 // A holds one row per (walker, electron, channel); Bt is a layer weight stored output-major (K
-// contiguous), so both operands stream K-contiguous vectors.  Tiles are 128 x 128 per 256-thread
-// workgroup (4 waves in 2 x 2, each wave 64 x 64 = 2 x 2 MFMA 32x32 blocks).  Tiles are assigned so
-// that the N-tiles of one M-tile run back to back on the same XCD (shared L2 for the A rows).
+// contiguous), so both operands stream K-contiguous vectors.  Tiles are up to 128 x 128 per 256-thread
+// workgroup (4 waves in 2 x 2).  Tiles are assigned so that the N-tiles of one M-tile run back to back on
+// the same XCD (shared L2 for the A rows).
 //
-// FP32 form: v_mfma_f32_32x32x2_f32.  BF16 forms: v_mfma_f32_32x32x16_bf16 with FP32 accumulation;
+// FP32 form: v_mfma_f32_16x16x4_f32.  BF16 forms: v_mfma_f32_32x32x16_bf16 with FP32 accumulation;
 // A is split into hi + lo BF16 parts while it is staged into LDS, the weights are pre-split, and the
 // x3 form accumulates hi*hi + hi*lo + lo*hi.
 #include "common.h"
@@ -57,19 +57,26 @@ __device__ __forceinline__ int tile_index(int bid, int total) {
 
 // ------------------------------------------------------------------------------------------------
 // FP32 MFMA.  Tile (64 WM) x (64 WN) per 256-thread workgroup; 4 waves in 2 x 2, each wave owns
-// WM x WN blocks of 32 x 32 (v_mfma_f32_32x32x2_f32).  WM = WN = 2 for large GEMMs; smaller tiles give
-// enough workgroups for the per-walker and orbital GEMMs of a value pass.
+// (2 WM) x (2 WN) blocks of 16 x 16 (v_mfma_f32_16x16x4_f32), so even the 64 x 64 tile keeps four
+// independent accumulators per wave (32 x 32 blocks would leave it one dependent chain).  WM = WN = 2 for
+// large GEMMs; smaller tiles give enough workgroups for the per-walker and orbital GEMMs of a value pass.
+// Every tile shape adds the same products to an output element in the same order, so the tile choice
+// never changes a result.  Both operand slices (16 k) are staged
+// untransposed in LDS with a row pitch of 20 floats; MFMA step s of lane group g = lane / 16 takes
+// k = 4 g + s, so a lane reads its operands for a whole slice as one 16-byte read per block (the 16 rows
+// one read touches start 20 floats apart and cover disjoint banks) and waits on LDS once per slice.
 // ------------------------------------------------------------------------------------------------
 template <int FLAGS, int WM, int WN>
 __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ A, int lda, const float* __restrict__ Bt,
                                                   int ldb, float* __restrict__ Cm, int ldc, int M, int N, int K,
                                                   int tilesN, int tilesTotal, GemmEpi ep) {
-  constexpr int BM = 64 * WM, BN = 64 * WN, BK = 16, PAD = 4;  // 32 measured no faster
+  constexpr int BM = 64 * WM, BN = 64 * WN, BK = 16, PITCH = BK + 4;
   constexpr int KV = BK / 4;  // float4 per row of a K-slice
-  constexpr int AV = BM * BK / 4, BV = BN * BK / 4;  // float4 per tile
+  constexpr int AV = BM * KV, BV = BN * KV;  // float4 per tile
   constexpr int AIT = (AV + 255) / 256, BIT = (BV + 255) / 256;
-  __shared__ float As[BK][BM + PAD];
-  __shared__ float Bs[BK][BN + PAD];
+  constexpr int MB = 2 * WM, NB = 2 * WN;  // 16 x 16 blocks per wave
+  __shared__ __align__(16) float As[BM][PITCH];
+  __shared__ __align__(16) float Bs[BN][PITCH];
   // split-K: workgroup range [split * gridTiles, (split + 1) * gridTiles) computes K slice `split`
   const int gridTiles = (tilesTotal + 7) / 8 * 8;
   const int split = blockIdx.x / gridTiles;
@@ -84,58 +91,45 @@ __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ A, i
   const int col0 = (tile % tilesN) * BN;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int wm = wid >> 1, wn = wid & 1;
+  const int g = lane >> 4, cl = lane & 15;
 
-  v16f acc[WM][WN];
+  v4f acc[MB][NB];
 #pragma unroll
-  for (int a = 0; a < WM; ++a)
+  for (int a = 0; a < MB; ++a)
 #pragma unroll
-    for (int b = 0; b < WN; ++b)
-#pragma unroll
-      for (int j = 0; j < 16; ++j) acc[a][b][j] = 0.f;
+    for (int b = 0; b < NB; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
 
-  float4 ra[AIT], rb[BIT];
+  // staging: vector index idx -> row 16 (idx / 64) + idx % 16, k quad (idx / 16) % 4, so the 16 lanes of
+  // one store phase write 16 rows at one quad (the banks of the 16-byte reads: no conflicts either way)
+  auto srow = [](int idx) { return ((idx >> 6) << 4) + (idx & 15); };
+  auto squad = [](int idx) { return ((idx >> 4) & 3) * 4; };
+  v4f ra[AIT], rb[BIT];
   auto gload = [&](int k0) {
 #pragma unroll
     for (int it = 0; it < AIT; ++it) {
       const int idx = tid + 256 * it;
       if (idx < AV) {
-        const int m = idx / KV, kq = (idx % KV) * 4;
-        long gr = row0 + m;
+        long gr = row0 + srow(idx);
         if (gr > M - 1) gr = M - 1;
-        ra[it] = *(const float4*)(A + gr * lda + k0 + kq);
+        ra[it] = *(const v4f*)(A + gr * lda + k0 + squad(idx));
       }
     }
 #pragma unroll
     for (int it = 0; it < BIT; ++it) {
       const int idx = tid + 256 * it;
-      if (idx < BV) {
-        const int m = idx / KV, kq = (idx % KV) * 4;
-        rb[it] = *(const float4*)(Bt + (long)(col0 + m) * ldb + k0 + kq);
-      }
+      if (idx < BV) rb[it] = *(const v4f*)(Bt + (long)(col0 + srow(idx)) * ldb + k0 + squad(idx));
     }
   };
   auto sstore = [&]() {
 #pragma unroll
     for (int it = 0; it < AIT; ++it) {
       const int idx = tid + 256 * it;
-      if (idx < AV) {
-        const int m = idx / KV, kq = (idx % KV) * 4;
-        As[kq + 0][m] = ra[it].x;
-        As[kq + 1][m] = ra[it].y;
-        As[kq + 2][m] = ra[it].z;
-        As[kq + 3][m] = ra[it].w;
-      }
+      if (idx < AV) *(v4f*)&As[srow(idx)][squad(idx)] = ra[it];
     }
 #pragma unroll
     for (int it = 0; it < BIT; ++it) {
       const int idx = tid + 256 * it;
-      if (idx < BV) {
-        const int m = idx / KV, kq = (idx % KV) * 4;
-        Bs[kq + 0][m] = rb[it].x;
-        Bs[kq + 1][m] = rb[it].y;
-        Bs[kq + 2][m] = rb[it].z;
-        Bs[kq + 3][m] = rb[it].w;
-      }
+      if (idx < BV) *(v4f*)&Bs[srow(idx)][squad(idx)] = rb[it];
     }
   };
 
@@ -145,34 +139,33 @@ __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ A, i
     sstore();
     __syncthreads();
     if (k0 + BK < K) gload(k0 + BK);
+    v4f a4[MB], b4[NB];
 #pragma unroll
-    for (int kk = 0; kk < BK; kk += 2) {
-      const int kr = kk + (lane >> 5);
-      float a[WM], b[WN];
+    for (int i = 0; i < MB; ++i) a4[i] = *(const v4f*)&As[wm * 32 * WM + i * 16 + cl][4 * g];
 #pragma unroll
-      for (int i = 0; i < WM; ++i) a[i] = As[kr][wm * 32 * WM + i * 32 + (lane & 31)];
+    for (int j = 0; j < NB; ++j) b4[j] = *(const v4f*)&Bs[wn * 32 * WN + j * 16 + cl][4 * g];
 #pragma unroll
-      for (int j = 0; j < WN; ++j) b[j] = Bs[kr][wn * 32 * WN + j * 32 + (lane & 31)];
+    for (int st = 0; st < 4; ++st)
 #pragma unroll
-      for (int i = 0; i < WM; ++i)
+      for (int i = 0; i < MB; ++i)
 #pragma unroll
-        for (int j = 0; j < WN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], b[j], acc[i][j], 0, 0, 0);
-    }
+        for (int j = 0; j < NB; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[i][st], b4[j][st], acc[i][j], 0, 0, 0);
   }
 
 #pragma unroll
-  for (int mi = 0; mi < WM; ++mi)
+  for (int i = 0; i < MB; ++i)
 #pragma unroll
-    for (int ni = 0; ni < WN; ++ni)
+    for (int j = 0; j < NB; ++j)
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        const int r = (int)row0 + wm * 32 * WM + mi * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-        const int c = col0 + wn * 32 * WN + ni * 32 + (lane & 31);
+      for (int e = 0; e < 4; ++e) {
+        const int r = (int)row0 + wm * 32 * WM + i * 16 + 4 * g + e;
+        const int c = col0 + wn * 32 * WN + j * 16 + cl;
         if (r < M && c < N) {
           if (split == 0)
-            epi_store<FLAGS, float>(acc[mi][ni][j], r, c, Cm, ldc, ep);
+            epi_store<FLAGS, float>(acc[i][j][e], r, c, Cm, ldc, ep);
           else
-            Cm[(long)r * ldc + c] = acc[mi][ni][j];
+            Cm[(long)r * ldc + c] = acc[i][j][e];
         }
       }
 }

@@ -271,7 +271,7 @@ __device__ __forceinline__ int mfma_row(int q, int h) { return 8 * (q >> 2) + 4 
 template <bool LAP, int L, int EPW>
 __global__ __launch_bounds__(64) void k_twoel_mfma(const float* __restrict__ r, TwoElOut<float> out,
                                                    const float* __restrict__ W2e, const float* __restrict__ b2e,
-                                                   Shape sh) {
+                                                   Shape sh, int tpw) {
   constexpr int NCH = LAP ? 5 : 1;
   constexpr int NL2 = L - 1;  // 2-el layers
   constexpr int NW = NL2 > 1 ? NL2 - 1 : 1;
@@ -280,9 +280,7 @@ __global__ __launch_bounds__(64) void k_twoel_mfma(const float* __restrict__ r, 
   const int lane = threadIdx.x, h = lane >> 5, fl = lane & 31;
   const int n = sh.n, u = sh.u, C = sh.C, Cp = sh.Cp;
   const int groups = (n + EPW - 1) / EPW;
-  const long b = blockIdx.x / groups;
-  const int i0 = (int)(blockIdx.x - b * groups) * EPW;  // first electron of the group
-  const float* rb = r + b * n * 3;
+  const long ntiles = (long)sh.B * groups;
   const float inv_up = 1.f / (float)u, inv_dn = 1.f / (float)(n - u);
   const float rs2 = 0.70710678118654752440f;
 
@@ -297,6 +295,11 @@ __global__ __launch_bounds__(64) void k_twoel_mfma(const float* __restrict__ r, 
 #pragma unroll
   for (int l2 = 0; l2 < NL2; ++l2) bias[l2] = b2e[l2 * 32 + fl];
 
+  // tpw consecutive (walker, electron group) tiles per wavefront share the weight loads above
+  for (long tile = (long)blockIdx.x * tpw, tile_end = tile + tpw; tile < tile_end && tile < ntiles; ++tile) {
+  const long b = tile / groups;
+  const int i0 = (int)(tile - b * groups) * EPW;  // first electron of the group
+  const float* rb = r + b * n * 3;
   // per-lane spin sums of the layer outputs, per electron of the group
   float su[NL2 > 0 ? NL2 : 1][EPW][NCH], sd[NL2 > 0 ? NL2 : 1][EPW][NCH];
   float ru[NCH][2], rd[NCH][2];  // raw pair features of this lane's row (k = h, 2 + h)
@@ -508,6 +511,7 @@ __global__ __launch_bounds__(64) void k_twoel_mfma(const float* __restrict__ r, 
       }
     }
   }
+  }  // tiles
 }
 
 template <bool LAP, int L>
@@ -515,19 +519,27 @@ static void twoel_mfma_dispatch(const float* r, const TwoElOut<float>& o, const 
                                 const Shape& sh, hipStream_t s) {
   // value pass: several electrons per wavefront for small n; the 5-channel Laplacian pass keeps one
   // electron per wavefront (its per-electron sums would not fit in registers)
+  // tiles per wavefront (weights loaded once per wavefront): 4 in the Laplacian pass (544 -> 512 us at
+  // 2048 walkers x 14 electrons); no gain measured in the value pass
+  const int tpw_v = 1;
+  int tpw_l = 4;
   if constexpr (!LAP) {
     const int epw = sh.n <= 8 ? 4 : (sh.n <= 16 ? 2 : 1);
-    const unsigned grid = (unsigned)(sh.B * ((sh.n + epw - 1) / epw));
+    const long tiles = (long)sh.B * ((sh.n + epw - 1) / epw);
+    const unsigned grid = (unsigned)((tiles + tpw_v - 1) / tpw_v);
     if (epw == 4) {
-      hipLaunchKernelGGL((k_twoel_mfma<false, L, 4>), dim3(grid), dim3(64), 0, s, r, o, W2e, b2e, sh);
+      hipLaunchKernelGGL((k_twoel_mfma<false, L, 4>), dim3(grid), dim3(64), 0, s, r, o, W2e, b2e, sh, tpw_v);
       return;
     }
     if (epw == 2) {
-      hipLaunchKernelGGL((k_twoel_mfma<false, L, 2>), dim3(grid), dim3(64), 0, s, r, o, W2e, b2e, sh);
+      hipLaunchKernelGGL((k_twoel_mfma<false, L, 2>), dim3(grid), dim3(64), 0, s, r, o, W2e, b2e, sh, tpw_v);
       return;
     }
+    tpw_l = tpw_v;
   }
-  hipLaunchKernelGGL((k_twoel_mfma<LAP, L, 1>), dim3((unsigned)(sh.B * sh.n)), dim3(64), 0, s, r, o, W2e, b2e, sh);
+  const long tiles = (long)sh.B * sh.n;
+  hipLaunchKernelGGL((k_twoel_mfma<LAP, L, 1>), dim3((unsigned)((tiles + tpw_l - 1) / tpw_l)), dim3(64), 0, s, r, o,
+                     W2e, b2e, sh, tpw_l);
 }
 
 template <bool LAP, typename T>
