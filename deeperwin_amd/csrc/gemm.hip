@@ -471,7 +471,12 @@ __global__ __launch_bounds__(256) void k_gemm_lap(const float* __restrict__ X, i
 
   const int Cp = sh.Cp, C = sh.C, n = sh.n;
   const long n_el_total = (long)sh.B * n;
-  const int tile_m = blockIdx.x / tilesN, tile_n = blockIdx.x % tilesN;
+  // XCD-aware order: the column tiles of a row tile are consecutive tiles of one XCD's range, so the row
+  // tile's second read of X is served by that XCD's L2 (the grid is a multiple of 8)
+  const long tiles_total = ((long)sh.B * sh.n + E - 1) / E * tilesN;
+  const int tile = tile_index(blockIdx.x, (int)tiles_total);
+  if (tile >= tiles_total) return;  // uniform over the workgroup, before any barrier
+  const int tile_m = tile / tilesN, tile_n = tile % tilesN;
   const long g0 = (long)tile_m * E;          // first electron of the tile
   const long row0 = g0 * Cp;
   const long rows_total = n_el_total * Cp;
@@ -756,7 +761,7 @@ static void gemm_lap_rb(int RB, const float* X, int ldx, const float* Bt, const 
     // 256-column tiles (every output column of a row tile in one workgroup, so its A tile is read once)
     if (cols == 256 && RB <= 6 && sh.W1 % 256 == 0) {
       const int tilesN = sh.W1 / 256;
-      const dim3 grid((unsigned)(tilesM * tilesN));
+      const dim3 grid((unsigned)((tilesM * tilesN + 7) / 8 * 8));
 #define DW_LAP_CASE256(R)                                                                                            \
   case R:                                                                                                            \
     hipLaunchKernelGGL((k_gemm_lap<R, 0, 256>), grid, dim3(256), 0, s, X, ldx, Bt, Bhi, Blo, K, V, ldv, out, ldo, \
@@ -775,7 +780,7 @@ static void gemm_lap_rb(int RB, const float* X, int ldx, const float* Bt, const 
     }
   }
   const int tilesN = sh.W1 / 128;
-  const dim3 grid((unsigned)(tilesM * tilesN));
+  const dim3 grid((unsigned)((tilesM * tilesN + 7) / 8 * 8));
 #define DW_LAP_CASE(R)                                                                                                  \
   case R:                                                                                                               \
     hipLaunchKernelGGL((k_gemm_lap<R, PREC, 128>), grid, dim3(256), 0, s, X, ldx, Bt, Bhi, Blo, K, V, ldv, out, ldo, \
@@ -808,7 +813,8 @@ void gemm_value16(const float* X, int ldx, const float* Bt, int K, const float* 
   const int E = 96;
   const int tilesN = sh.W1 / 128;
   const long tilesM = ((long)sh.B * sh.n + E - 1) / E;
-  hipLaunchKernelGGL((k_gemm_lap<6, 0, 128, true>), dim3((unsigned)(tilesM * tilesN)), dim3(256), 0, s, X, ldx, Bt,
+  const dim3 grid((unsigned)((tilesM * tilesN + 7) / 8 * 8));
+  hipLaunchKernelGGL((k_gemm_lap<6, 0, 128, true>), grid, dim3(256), 0, s, X, ldx, Bt,
                      nullptr, nullptr, K, V, ldv, out, ldo, permute, E, v, tilesN, vsplit, vstride);
 }
 

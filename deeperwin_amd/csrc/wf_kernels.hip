@@ -896,87 +896,89 @@ __device__ __forceinline__ void finish_slogdet(double mydiag, int swaps, bool si
   }
 }
 
-// Value mode: 64 / NMAX matrices per wavefront (lane group g holds matrix g, lane j of the group holds
-// column j), 4 wavefronts per workgroup, no LDS.  Pivot search in the pivot column's lane, the pivot
-// index, pivot and multipliers are broadcast inside the group with shuffles.
+// Value mode: 64 / NMAX matrices per wavefront (lane group g holds matrix g), 4 wavefronts per workgroup.
+// The matrix is formed with a lane per column (coalesced reads of O), staged in wave-private LDS, and
+// factorised with a lane per row: a row interchange is then a relabelling (no data moves), the pivot is a
+// butterfly arg-max over the group's lanes (largest |a|, lowest row on ties), and each remaining row takes
+// the pivot row's entries by shuffle.  Multipliers a[i][p] * (1 / pivot) (the reciprocal, as LAPACK's
+// dgetf2 scales) and updates a[i][c] -= l a[piv][c] are the same operations as a column-oriented
+// elimination, so U, log|det| (summed over U's diagonal in a fixed lane order) and the conditioning are
+// too; the sign counts the negative pivots and the parity of the pivot order.
 template <int NMAX, typename TA>
 __global__ __launch_bounds__(256) void k_det_value(OrbSrc<TA> src, double* __restrict__ ld, double* __restrict__ sd,
                                                    float* __restrict__ dcond, long nmat, int n, Shape sh) {
   constexpr int G = 64 / NMAX;
   const int lane = threadIdx.x & 63;
-  const int g = lane / NMAX, j = lane % NMAX, base = g * NMAX;
+  const int g = lane / NMAX, t = lane % NMAX, base = g * NMAX;
   const long bd = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * G + g;
   const bool valid = bd < nmat;
   const long bdl = valid ? bd : nmat - 1;  // out-of-range groups factor a copy, never stored
   const long b = bdl / sh.D;
-  const int oc = (int)(bdl - b * sh.D) * n + j;  // orbital column of matrix column j
-  const bool act = j < n;
-  // matrix element (i, j) = env(r_i, column) * O[electron i, column]: the envelope is applied as O is read.
-  // Each lane forms its column in a rolled loop into wave-private LDS (lane-contiguous, conflict-free) and
-  // reads it back into registers: the element phase stays out of the unrolled factorisation.
+  const int oc = (int)(bdl - b * sh.D) * n + t;  // orbital column of matrix column t
+  const bool act = t < n;                         // column t (forming), row t (factorising) exists
+  // element (r, c) = env(r_r, column c) * O[electron r, column c]: the envelope is applied as O is read.
+  // Lane t forms column t in a rolled loop (the element phase stays out of the unrolled factorisation);
+  // element (r, c) of group g sits at slot base + (c ^ r) of LDS row r, so both this write (fixed r) and
+  // the row read below (fixed c, lanes over r) touch distinct banks.
   __shared__ TA sA[4][NMAX][64];
   TA (*mine)[64] = sA[threadIdx.x >> 6];
 #pragma unroll 1
-  for (int i = 0; i < n; ++i) {
+  for (int r = 0; r < n; ++r) {
     TA v = TA(0);
     if (act) {
       TA env, g0, g1, g2, lv;
-      envelope_at<TA, false>(src.kappa, src.wenv, src.r, src.R, sh, b, i, oc, env, g0, g1, g2, lv);
-      v = env * src.O[orb_row(sh, b, i) * src.ldo + oc];
+      envelope_at<TA, false>(src.kappa, src.wenv, src.r, src.R, sh, b, r, oc, env, g0, g1, g2, lv);
+      v = env * src.O[orb_row(sh, b, r) * src.ldo + oc];
     }
-    mine[i][lane] = v;
+    mine[r][base + (t ^ r)] = v;
   }
-  double col[NMAX];
+  __syncthreads();  // every wavefront runs the same iterations: no early exit above
+  double a[NMAX];  // row t
 #pragma unroll
-  for (int i = 0; i < NMAX; ++i) col[i] = (act && i < n) ? (double)mine[i][lane] : 0.0;
-  int swaps = 0;
+  for (int c = 0; c < NMAX; ++c) a[c] = (act && c < n) ? (double)mine[t][base + (c ^ t)] : 0.0;
+  bool used = !act;  // padding rows never pivot
+  int step_of = 0;   // elimination step at which this row became the pivot row
   bool sing = false;
-  double mydiag = 1.0;
+  double mydiag = 1.0;  // lane p: U[p][p]
 #pragma unroll
   for (int p = 0; p < NMAX; ++p) {
     if (p < n) {
-      double best = -1.0;
-      int bidx = p;
+      const double ap = fabs(a[p]);
+      double v = (used || !(ap >= 0.0)) ? -1.0 : ap;
+      int idx = t;
 #pragma unroll
-      for (int i = p; i < NMAX; ++i) {
-        if (i < n) {
-          const double a = fabs(col[i]);
-          if (a > best) {
-            best = a;
-            bidx = i;
-          }
+      for (int o = NMAX / 2; o > 0; o >>= 1) {
+        const double v2 = __shfl_xor(v, o, NMAX);
+        const int i2 = __shfl_xor(idx, o, NMAX);
+        if (v2 > v || (v2 == v && i2 < idx)) {
+          v = v2;
+          idx = i2;
         }
       }
-      const int piv = __shfl(bidx, base + p, 64);
-      const double bval = __shfl(best, base + p, 64);
-      if (!(bval > 0.0)) sing = true;
-      if (piv != p) {
-        ++swaps;
-#pragma unroll
-        for (int i = p + 1; i < NMAX; ++i) {
-          if (i == piv) {
-            const double t = col[p];
-            col[p] = col[i];
-            col[i] = t;
-          }
-        }
+      if (!(v > 0.0)) sing = true;
+      const double pivot = __shfl(a[p], base + idx, 64);
+      if (t == p) mydiag = pivot;
+      if (t == idx) {
+        used = true;
+        step_of = p;
       }
-      const double pivot = __shfl(col[p], base + p, 64);
-      if (j == p) {
-        mydiag = col[p];
-        const double rp = 1.0 / pivot;  // multipliers by the reciprocal, as LAPACK's dgetf2 scales
+      const double rp = 1.0 / pivot;
+      const double l = a[p] * rp;
 #pragma unroll
-        for (int i = p + 1; i < NMAX; ++i)
-          if (i < n) col[i] = col[i] * rp;
-      }
-#pragma unroll
-      for (int i = p + 1; i < NMAX; ++i) {
-        if (i < n) {
-          const double li = __shfl(col[i], base + p, 64);
-          if (j > p) col[i] -= li * col[p];
+      for (int c = p + 1; c < NMAX; ++c) {
+        if (c < n) {
+          const double pc = __shfl(a[c], base + idx, 64);
+          if (!used) a[c] -= l * pc;
         }
       }
     }
+  }
+  // parity of the pivot order: inversions of step_of over the rows
+  int inv = 0;
+#pragma unroll
+  for (int c = 0; c < NMAX; ++c) {
+    const int sc = __shfl(step_of, base + c, 64);
+    if (act && c > t && c < n && sc < step_of) ++inv;
   }
   double lg = act ? log(fabs(mydiag)) : 0.0;
   int neg = (act && mydiag < 0.0) ? 1 : 0;
@@ -985,12 +987,13 @@ __global__ __launch_bounds__(256) void k_det_value(OrbSrc<TA> src, double* __res
   for (int o = NMAX / 2; o > 0; o >>= 1) {
     lg += __shfl_xor(lg, o, NMAX);
     neg += __shfl_xor(neg, o, NMAX);
+    inv += __shfl_xor(inv, o, NMAX);
     umax = fmax(umax, __shfl_xor(umax, o, NMAX));
     umin = fmin(umin, __shfl_xor(umin, o, NMAX));
   }
-  if (j == 0 && valid) {
+  if (t == 0 && valid) {
     ld[bd] = sing ? -INFINITY : lg;
-    sd[bd] = sing ? 0.0 : (((neg + swaps) & 1) ? -1.0 : 1.0);
+    sd[bd] = sing ? 0.0 : (((neg + inv) & 1) ? -1.0 : 1.0);
     dcond[bd] = (float)(umax / umin);
   }
 }
