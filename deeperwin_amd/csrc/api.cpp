@@ -584,7 +584,14 @@ dw_status run_pass(dw_model* m, const PassBufs<T>& ws, const float* r, long B, c
       launch_dpe4_tanh<T>(ws.HM, W.bhm[l].as<T>(), sh, s);
       launch_dpe4_conv<T>(ws.WM + l * wm_layer, ws.HM, X[l], m->Kx[l], m->Kt0[l], sh, s);
     }
-    launch_spin_mean<T>(X[l], m->Kx[l], m->Kh[l], ws.MU, m->Km[l], sh, s);
+    // Laplacian pass, FP32 form: spin means formed inside the per-walker GEMM (same values as below)
+    bool fused_mu = false;
+    if constexpr (!DBL) {
+      if (C > 1 && m->prec == GEMM_F32)
+        fused_mu = gemm_mean_lap(X[l], m->Kx[l], m->Kh[l], W.Wmu[l].as<float>(), m->Km[l], W.bias[l].as<float>(),
+                                 ws.V, m->W1, sh, s);
+    }
+    if (!fused_mu) launch_spin_mean<T>(X[l], m->Kx[l], m->Kh[l], ws.MU, m->Km[l], sh, s);
     // value pass: split K of the per-walker spin-mean GEMM so that it fills the chip (its M is only B)
     int vs = 1;
     if (!DBL && C == 1 && m->prec == GEMM_F32) {
@@ -603,8 +610,9 @@ dw_status run_pass(dw_model* m, const PassBufs<T>& ws, const float* r, long B, c
     ev.ksplit = vs;
     ev.cstride = vstride;
     ev.tile = C == 1 ? -1 : m->tune.lap_mu_tile;
-    gemm_t(m, ws.MU, m->Km[l], W.Wmu[l].as<T>(), W.Wmu_hi[l], W.Wmu_lo[l], m->Km[l], ws.V, m->W1,
-           B * Cp, m->W1, m->Km[l], ev, s);
+    if (!fused_mu)
+      gemm_t(m, ws.MU, m->Km[l], W.Wmu[l].as<T>(), W.Wmu_hi[l], W.Wmu_lo[l], m->Km[l], ws.V, m->W1,
+             B * Cp, m->W1, m->Km[l], ev, s);
     const bool last = (l == m->L - 1);
     T* outp = last ? ws.H : X[l + 1];
     const int ldo = last ? m->W1 : m->Kx[l + 1];

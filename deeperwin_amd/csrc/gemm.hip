@@ -818,6 +818,116 @@ void gemm_value16(const float* X, int ldx, const float* Bt, int K, const float* 
                      nullptr, nullptr, K, V, ldv, out, ldo, permute, E, v, tilesN, vsplit, vstride);
 }
 
+// ------------------------------------------------------------------------------------------------
+// Per-walker term of a 1-el layer in the Laplacian pass (FP32), spin means fused in:
+//   V[r, o] = sum_k MU[r, k] Wmu[o, k] (+ bias[o] on each walker's value channel),
+//   MU[r, k] = mean over the electrons of spin s(k) of X[(b n + i) Cp + ch, k - s Kh],  r = b Cp + ch,
+// with MU formed in LDS for 48 rows and 256 of its columns at a time (never stored).  The means add the
+// electrons in order and scale by 1 / n_s like k_spin_mean, and the MFMA sequence per output element
+// (16-wide K slices in order, step s of lane group g taking k = 4 g + s) is k_gemm_f32's, so V equals
+// what those two kernels produce.  4 waves x 64 output columns (N = 256); each wave's weights come
+// straight from global memory (L2), one slice ahead.
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mean_gemm(const float* __restrict__ X, int Kx, int Kh,
+                                                   const float* __restrict__ Bt, int Km, const float* __restrict__ bias,
+                                                   float* __restrict__ V, int ldv, Shape sh) {
+  constexpr int RT = 48, RB = RT / 16, CW = 256, PITCH = CW + 4;
+  __shared__ __align__(16) float Ms[RT][PITCH];
+  const long rows = (long)sh.B * sh.Cp;
+  const long r0 = (long)blockIdx.x * RT;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, g = lane >> 4, cl = lane & 15;
+  const int cw = wv * 64;  // this wave's first output column
+  const int n = sh.n, u = sh.u, Cp = sh.Cp;
+  const long xstride = (long)Cp * Kx;  // one electron further
+
+  v4f acc[RB][4];
+#pragma unroll
+  for (int a = 0; a < RB; ++a)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[a][q] = v4f{0.f, 0.f, 0.f, 0.f};
+
+  for (int kc = 0; kc < Km; kc += CW) {
+    const int cwid = Km - kc < CW ? Km - kc : CW;  // a multiple of 16
+    const int q4 = cwid / 4;
+    __syncthreads();  // the previous chunk's reads of Ms are done
+    for (int e = tid; e < RT * q4; e += 256) {
+      const int rr = e / q4, f = (e - rr * q4) * 4;
+      const long r = r0 + rr;
+      const int k = kc + f;
+      float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f;
+      if (r < rows && k < 2 * Kh) {
+        const long b = div_idx(r, Cp);
+        const int ch = (int)(r - b * Cp);
+        const int s = k >= Kh ? 1 : 0;
+        const int i0 = s ? u : 0, i1 = s ? n : u;
+        const float* xp = X + ((b * n + i0) * Cp + ch) * (long)Kx + (k - s * Kh);
+#pragma unroll 4
+        for (int i = i0; i < i1; ++i, xp += xstride) {
+          const v4f x = *(const v4f*)xp;
+          v0 += x[0];
+          v1 += x[1];
+          v2 += x[2];
+          v3 += x[3];
+        }
+        const float inv = 1.f / (float)(i1 - i0);
+        v0 *= inv;
+        v1 *= inv;
+        v2 *= inv;
+        v3 *= inv;
+      }
+      *(v4f*)&Ms[rr][f] = v4f{v0, v1, v2, v3};
+    }
+    __syncthreads();
+    v4f bq[4], bn[4];
+    auto bload = [&](int k0, v4f* b) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) b[q] = *(const v4f*)(Bt + (long)(cw + 16 * q + cl) * Km + kc + k0 + 4 * g);
+    };
+    bload(0, bq);
+    for (int k0 = 0; k0 < cwid; k0 += 16) {
+      if (k0 + 16 < cwid) bload(k0 + 16, bn);
+      v4f a4[RB];
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) a4[rb] = *(const v4f*)&Ms[rb * 16 + cl][k0 + 4 * g];
+#pragma unroll
+      for (int st = 0; st < 4; ++st)
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            acc[rb][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[rb][st], bq[q][st], acc[rb][q], 0, 0, 0);
+      if (k0 + 16 < cwid) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bq[q] = bn[q];
+      }
+    }
+  }
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const long r = r0 + rb * 16 + 4 * g + e;
+        if (r < rows) {
+          const int c = cw + 16 * q + cl;
+          float v = acc[rb][q][e];
+          if (bias && r - div_idx(r, Cp) * Cp == 0) v += bias[c];
+          V[r * ldv + c] = v;
+        }
+      }
+}
+
+bool gemm_mean_lap(const float* X, int Kx, int Kh, const float* Wmu, int Km, const float* bias, float* V, int ldv,
+                   const Shape& sh, hipStream_t s) {
+  // (the layer-0 GEMM, K of a few features, runs faster as the two separate kernels)
+  if (sh.W1 != 256 || Km < 256 || Km % 16 || Kh % 4 || Kx % 4 || sh.C < 2) return false;
+  const long rows = (long)sh.B * sh.Cp;
+  hipLaunchKernelGGL(k_mean_gemm, dim3((unsigned)((rows + 47) / 48)), dim3(256), 0, s, X, Kx, Kh, Wmu, Km, bias, V,
+                     ldv, sh);
+  return true;
+}
+
 int lap_electrons_per_tile(int Cp, GemmPrec prec) {
   // FP32: the most electrons (at most 4) whose rows fit 192; measured 1.7-2.4 % faster than 96 rows at
   // 4, 14, 17 and 26 electrons.  BF16 forms keep 96 rows.

@@ -57,6 +57,13 @@ int lap_electrons_per_tile(int Cp, GemmPrec prec);
 void gemm_value16(const float* X, int ldx, const float* Bt, int K, const float* V, int ldv, int vsplit, long vstride,
                   float* out, int ldo, int permute, const Shape& sh, hipStream_t s);
 
+// Laplacian-pass per-walker GEMM with the spin means of X's h columns formed in LDS (FP32):
+// V[r, o] = sum_k MU[r, k] Wmu[o, k] + bias[o] on value-channel rows, equal to launch_spin_mean followed by
+// the bias-only gemm on MU.  Returns false, launching nothing, unless W1 == 256, Km >= 256, Km % 16 == 0 and
+// Kh, Kx are multiples of 4 (C > 1).
+bool gemm_mean_lap(const float* X, int Kx, int Kh, const float* Wmu, int Km, const float* bias, float* V, int ldv,
+                   const Shape& sh, hipStream_t s);
+
 // Layer-0 input rows: the el-ion features in columns [0, 4 I); the columns [tail0, tail0 + tailw) are left
 // to the 2-el kernels for the real channel rows; every other column and padding row is zero.
 template <typename T>
