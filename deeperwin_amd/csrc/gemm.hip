@@ -66,11 +66,11 @@ __device__ __forceinline__ int tile_index(int bid, int total) {
 // k = 4 g + s, so a lane reads its operands for a whole slice as one 16-byte read per block (the 16 rows
 // one read touches start 20 floats apart and cover disjoint banks) and waits on LDS once per slice.
 // ------------------------------------------------------------------------------------------------
-template <int FLAGS, int WM, int WN>
+template <int FLAGS, int WM, int WN, int BK>
 __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ A, int lda, const float* __restrict__ Bt,
                                                   int ldb, float* __restrict__ Cm, int ldc, int M, int N, int K,
                                                   int tilesN, int tilesTotal, GemmEpi ep) {
-  constexpr int BM = 64 * WM, BN = 64 * WN, BK = 16, PITCH = BK + 4;
+  constexpr int BM = 64 * WM, BN = 64 * WN, PITCH = BK + 4;  // BK = 16 or 32
   constexpr int KV = BK / 4;  // float4 per row of a K-slice
   constexpr int AV = BM * KV, BV = BN * KV;  // float4 per tile
   constexpr int AIT = (AV + 255) / 256, BIT = (BV + 255) / 256;
@@ -99,10 +99,10 @@ __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ A, i
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
 
-  // staging: vector index idx -> row 16 (idx / 64) + idx % 16, k quad (idx / 16) % 4, so the 16 lanes of
-  // one store phase write 16 rows at one quad (the banks of the 16-byte reads: no conflicts either way)
-  auto srow = [](int idx) { return ((idx >> 6) << 4) + (idx & 15); };
-  auto squad = [](int idx) { return ((idx >> 4) & 3) * 4; };
+  // staging: vector index idx -> row 16 (idx / (16 KV)) + idx % 16, k quad (idx / 16) % KV, so the 16
+  // lanes of one store phase write 16 rows at one quad (the banks of the 16-byte reads: no conflicts)
+  auto srow = [](int idx) { return (idx / (16 * KV)) * 16 + (idx & 15); };
+  auto squad = [](int idx) { return ((idx >> 4) % KV) * 4; };
   v4f ra[AIT], rb[BIT];
   auto gload = [&](int k0) {
 #pragma unroll
@@ -139,18 +139,21 @@ __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ A, i
     sstore();
     __syncthreads();
     if (k0 + BK < K) gload(k0 + BK);
-    v4f a4[MB], b4[NB];
 #pragma unroll
-    for (int i = 0; i < MB; ++i) a4[i] = *(const v4f*)&As[wm * 32 * WM + i * 16 + cl][4 * g];
+    for (int kk = 0; kk < BK; kk += 16) {  // 16-wide sub-slices in order: the same sequence for any BK
+      v4f a4[MB], b4[NB];
 #pragma unroll
-    for (int j = 0; j < NB; ++j) b4[j] = *(const v4f*)&Bs[wn * 32 * WN + j * 16 + cl][4 * g];
+      for (int i = 0; i < MB; ++i) a4[i] = *(const v4f*)&As[wm * 32 * WM + i * 16 + cl][kk + 4 * g];
 #pragma unroll
-    for (int st = 0; st < 4; ++st)
+      for (int j = 0; j < NB; ++j) b4[j] = *(const v4f*)&Bs[wn * 32 * WN + j * 16 + cl][kk + 4 * g];
 #pragma unroll
-      for (int i = 0; i < MB; ++i)
+      for (int st = 0; st < 4; ++st)
 #pragma unroll
-        for (int j = 0; j < NB; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[i][st], b4[j][st], acc[i][j], 0, 0, 0);
+        for (int i = 0; i < MB; ++i)
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[i][st], b4[j][st], acc[i][j], 0, 0, 0);
+    }
   }
 
 #pragma unroll
@@ -373,8 +376,15 @@ static void launch_f32(const float* A, int lda, const float* Bt, int ldb, float*
   const int tilesM = (M + 64 * WM - 1) / (64 * WM), tilesN = (N + 64 * WN - 1) / (64 * WN);
   const int total = tilesM * tilesN;
   const int grid = (total + 7) / 8 * 8 * ep.ksplit;
-  hipLaunchKernelGGL((k_gemm_f32<FLAGS, WM, WN>), dim3(grid), dim3(256), 0, s, A, lda, Bt, ldb, C, ldc, M, N, K,
-                     tilesN, total, ep);
+  // 32-wide K slices on 128 x 128 tiles (half the barriers per MFMA; orbital GEMMs of the N2 Laplacian pass
+  // 940 -> 845 us); the small tiles measured faster with 16 (N2 value pass 448 vs 485 us).  Results are the
+  // same either way: a slice runs as 16-wide sub-slices in order.
+  if (WM == 2 && WN == 2 && (K / ep.ksplit) % 32 == 0)
+    hipLaunchKernelGGL((k_gemm_f32<FLAGS, WM, WN, 32>), dim3(grid), dim3(256), 0, s, A, lda, Bt, ldb, C, ldc, M, N,
+                       K, tilesN, total, ep);
+  else
+    hipLaunchKernelGGL((k_gemm_f32<FLAGS, WM, WN, 16>), dim3(grid), dim3(256), 0, s, A, lda, Bt, ldb, C, ldc, M, N,
+                       K, tilesN, total, ep);
 }
 
 // Workgroups needed to fill the chip a few times over: the largest tile that still gives >= 4 per CU.
