@@ -284,14 +284,25 @@ __global__ __launch_bounds__(64) void k_twoel_mfma(const float* __restrict__ r, 
   const float inv_up = 1.f / (float)u, inv_dn = 1.f / (float)(n - u);
   const float rs2 = 0.70710678118654752440f;
 
-  // B operands: lane holds W[k = 2 s + h][f = fl]
-  float w0[2], wB[NW][16], bias[NL2 > 0 ? NL2 : 1];
+  // B operands: lane holds W[k = 2 s + h][f = fl].  The value pass keeps the 32 -> 32 layers' weights in
+  // LDS (read per MFMA step), which lowers its registers to 3 wavefronts per SIMD; the 5-channel Laplacian
+  // pass keeps them in registers (one wavefront per SIMD either way).
+  constexpr bool WLDS = !LAP;
+  __shared__ float wsh[WLDS ? NW : 1][WLDS ? 32 : 1][WLDS ? 32 : 1];
+  float w0[2], wB[WLDS ? 1 : NW][16], bias[NL2 > 0 ? NL2 : 1];
 #pragma unroll
   for (int q = 0; q < 2; ++q) w0[q] = W2e[(2 * q + h) * 32 + fl];
 #pragma unroll
   for (int l2 = 1; l2 < NL2; ++l2)
 #pragma unroll
-    for (int q = 0; q < 16; ++q) wB[l2 - 1][q] = W2e[(l2 * 32 + 2 * q + h) * 32 + fl];
+    for (int q = 0; q < 16; ++q) {
+      const float w = W2e[(l2 * 32 + 2 * q + h) * 32 + fl];
+      if constexpr (WLDS)
+        wsh[l2 - 1][2 * q + h][fl] = w;
+      else
+        wB[l2 - 1][q] = w;
+    }
+  if constexpr (WLDS) __syncthreads();
 #pragma unroll
   for (int l2 = 0; l2 < NL2; ++l2) bias[l2] = b2e[l2 * 32 + fl];
 
@@ -383,7 +394,9 @@ __global__ __launch_bounds__(64) void k_twoel_mfma(const float* __restrict__ r, 
         for (int q = 0; q < 16; ++q)
 #pragma unroll
           for (int c = 0; c < NCH; ++c)
-            acc[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(xs[c][fl][2 * q + h], wB[l2 - 1][q], acc[c], 0, 0, 0);
+            acc[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(xs[c][fl][2 * q + h],
+                                                          WLDS ? wsh[l2 - 1][2 * q + h][fl] : wB[WLDS ? 0 : l2 - 1][q],
+                                                          acc[c], 0, 0, 0);
       }
       // tanh with the forward Laplacian rule (in place), then the residual (x + y) / sqrt 2 on the
       // 32 -> 32 layers with x read back from the transposition buffer (it still holds the layer input)
