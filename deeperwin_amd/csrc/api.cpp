@@ -552,12 +552,19 @@ void gemm_t(const dw_model*, const double* A, int lda, const double* Bt, const D
   gemm_f64(A, lda, Bt, ldb, C, ldc, (int)M, N, K, ep, s);
 }
 
+// Gaussian proposal of an MCMC step, formed inside a value pass: r' = r + stepsize * normals -> r_out.
+struct Proposal {
+  const float* normals;
+  const void* chain;
+  float* r_out;
+};
+
 // One pass of the wavefunction over B walkers with C channels (C == 1: value; C == 3n+2: Laplacian).
 // `ws` is the pass's scratch (layout_pass); out_kappa (may be null) receives the per-walker conditioning.
 template <typename T>
 dw_status run_pass(dw_model* m, const PassBufs<T>& ws, const float* r, long B, const float* R, const float* Z, int C,
                    hipStream_t s, float* out_lp, float* out_phase, float* out_eloc, uint32_t* out_flags,
-                   float* out_kappa, float cond_thr) {
+                   float* out_kappa, float cond_thr, const Proposal* prop = nullptr) {
   constexpr bool DBL = sizeof(T) == 8;
   Weights& W = DBL ? m->wd : m->wf;
   const int Cp = rows_per_electron(C);
@@ -565,7 +572,12 @@ dw_status run_pass(dw_model* m, const PassBufs<T>& ws, const float* r, long B, c
   const long rows = B * m->n * Cp;
   Timed pass_timer(m, C > 1 ? DW_TIMER_LOCAL_ENERGY : DW_TIMER_FORWARD, s);
   T* const* X = ws.X;
-  launch_features0<T>(r, R, X[0], m->Kx[0], m->Kt0[0], m->Kt[0], sh, s);
+  if (prop) {  // the pass runs on the Gaussian proposal, formed and stored by the features kernel
+    launch_features0<T>(r, R, X[0], m->Kx[0], m->Kt0[0], m->Kt[0], sh, s, prop->normals, prop->chain, prop->r_out);
+    r = prop->r_out;
+  } else {
+    launch_features0<T>(r, R, X[0], m->Kx[0], m->Kt0[0], m->Kt[0], sh, s);
+  }
   if (m->dpe4) {
     launch_dpe4_pairs<T>(r, W.Wp.as<T>(), W.bp.as<T>(), ws.WM, sh, s);
     launch_dpe4_elion<T>(r, R, Z, W.Wei.as<T>(), W.bei.as<T>(), W.Whim.as<T>(), W.bhim.as<T>(), W.table.as<T>(),
@@ -950,7 +962,7 @@ dw_status mcmc_run(dw_model* m, const McmcArgs& a, const FixedScratch* fx, hipSt
     rprop = cv.take<float>((size_t)cnt3);
     lpprop = cv.take<float>((size_t)B);
     if (scaled) logq = cv.take<float>((size_t)B);
-    cnt = cv.take<int32_t>((size_t)std::max(a.n_steps, 1));
+    cnt = cv.take<int32_t>((size_t)2 * std::max(a.n_steps, 1));  // accepted counts, then tickets
     if (cv.off > scratch_bytes) {
       const size_t per_walker = (pass_bytes(m, 64, 1, dbl) + 64 * (size_t)(m->n * 3 + 2) * 4) / 64;
       return fail(DW_ERR_INVALID, "dw_mcmc_steps_noalloc: " + std::to_string(B) + " walkers need " +
@@ -972,7 +984,7 @@ dw_status mcmc_run(dw_model* m, const McmcArgs& a, const FixedScratch* fx, hipSt
     if (st) return st;
     HIP_TRY(m->rprop.ensure((size_t)cnt3 * 4));
     HIP_TRY(m->lpprop.ensure((size_t)B * 4));
-    HIP_TRY(m->cnt.ensure((size_t)std::max(a.n_steps, 1) * 4));
+    HIP_TRY(m->cnt.ensure((size_t)2 * std::max(a.n_steps, 1) * 4));  // accepted counts, then tickets
     rprop = m->rprop.as<float>();
     lpprop = m->lpprop.as<float>();
     cnt = m->cnt.as<int32_t>();
@@ -986,14 +998,15 @@ dw_status mcmc_run(dw_model* m, const McmcArgs& a, const FixedScratch* fx, hipSt
   // the timing hooks create events; the non-allocating variant records none
   const bool timing = m->timing;
   if (fx) m->timing = false;
-  auto value_pass = [&](const float* rin, float* lp_out) {
-    return dbl ? run_pass<double>(m, pd, rin, B, a.R, a.Z, 1, s, lp_out, nullptr, nullptr, nullptr, nullptr, 0.f)
-               : run_pass<float>(m, pf, rin, B, a.R, a.Z, 1, s, lp_out, nullptr, nullptr, nullptr, nullptr, 0.f);
+  auto value_pass = [&](const float* rin, float* lp_out, const Proposal* prop) {
+    if (dbl)
+      return run_pass<double>(m, pd, rin, B, a.R, a.Z, 1, s, lp_out, nullptr, nullptr, nullptr, nullptr, 0.f, prop);
+    return run_pass<float>(m, pf, rin, B, a.R, a.Z, 1, s, lp_out, nullptr, nullptr, nullptr, nullptr, 0.f, prop);
   };
   // log_psi_sqr is re-evaluated on entry (mcmc.py:396)
-  dw_status st = value_pass(a.r, a.log_psi_sqr);
+  dw_status st = value_pass(a.r, a.log_psi_sqr, nullptr);
   if (!st && a.n_steps > 0) {
-    launch_fill_i32(cnt, 0, a.n_steps, s);
+    launch_fill_i32(cnt, 0, 2 * a.n_steps, s);
     for (int k = 0; k < a.n_steps && !st; ++k) {
       const float* xi = a.normals + (size_t)k * draw_stride;
       if (one_el)
@@ -1002,14 +1015,14 @@ dw_status mcmc_run(dw_model* m, const McmcArgs& a, const FixedScratch* fx, hipSt
       else if (scaled)
         launch_propose_scaled(kind, a.r, xi, a.chain, a.R, a.Z, m->I, m->prop_rmin, m->prop_rmax, m->prop_scale, rprop,
                               logq, (int)B, m->n, s);
-      else
-        launch_propose(a.r, xi, a.chain, rprop, cnt3, s);
-      st = value_pass(rprop, lpprop);
+      // the Gaussian proposal is formed by the pass's first kernel, which stores it to rprop
+      const Proposal gauss{xi, a.chain, rprop};
+      st = (one_el || scaled) ? value_pass(rprop, lpprop, nullptr) : value_pass(a.r, lpprop, &gauss);
       if (st) break;
       launch_accept(a.r, rprop, a.log_psi_sqr, lpprop, logq, a.uniforms + (size_t)k * B, a.walker_age, a.max_age,
-                    cnt + k, a.out_flags, (int)B, m->n, s);
-      launch_chain_update(a.chain, cnt + k, (int)B, a.out_acc ? a.out_acc + k : nullptr, a.stepsize_update_interval,
-                          a.target_acceptance, a.min_stepsize, a.max_stepsize, s);
+                    cnt + k, a.out_flags, (int)B, m->n, a.chain, cnt + a.n_steps + k,
+                    a.out_acc ? a.out_acc + k : nullptr, a.stepsize_update_interval, a.target_acceptance,
+                    a.min_stepsize, a.max_stepsize, s);
     }
   }
   m->timing = timing;

@@ -65,10 +65,13 @@ bool gemm_mean_lap(const float* X, int Kx, int Kh, const float* Wmu, int Km, con
                    const Shape& sh, hipStream_t s);
 
 // Layer-0 input rows: the el-ion features in columns [0, 4 I); the columns [tail0, tail0 + tailw) are left
-// to the 2-el kernels for the real channel rows; every other column and padding row is zero.
+// to the 2-el kernels for the real channel rows; every other column and padding row is zero.  With normals
+// (and the device chain record), the rows are those of the Gaussian proposal r + stepsize * normals, which is
+// also written to r_out.
 template <typename T>
 void launch_features0(const float* r, const float* R, T* X0, int Kx0, int tail0, int tailw, const Shape& sh,
-                      hipStream_t s);
+                      hipStream_t s, const float* normals = nullptr, const void* chain = nullptr,
+                      float* r_out = nullptr);
 template <typename T>
 void launch_twoel(const float* r, T* const* X, const int* Kx, const int* Kh, const T* W2e, const T* b2e,
                   const Shape& sh, hipStream_t s);
@@ -145,10 +148,12 @@ void launch_propose_scaled(int kind, const float* r, const float* normals, const
 void launch_propose_one_el(int kind, const float* r, const float* xi, const void* chain, const float* R, int I,
                            float r_min, float r_max, float* r_prop, float* logq, int B, int n, hipStream_t s);
 // logq (may be null for a symmetric proposal) is added to the exponent: accept iff exp(lp' - lp + logq) > u.
+// The step's accepted walkers are added to *acc_count; the last workgroup (by *ticket, which must be zero
+// on entry) then updates the chain record: step_nr, acc_rate and, every `interval` steps, the stepsize from
+// the previous acc_rate, clipped to [lo, hi]; out_acc (may be null) receives the step's acceptance.
 void launch_accept(float* r, const float* r_prop, float* lp, const float* lp_prop, const float* logq,
                    const float* uniforms, int32_t* age, int max_age, int32_t* acc_count, uint32_t* flags, int B,
-                   int n, hipStream_t s);
-void launch_chain_update(void* chain, const int32_t* acc_count, int B, float* out_acc, int interval, float target,
-                         float lo, float hi, hipStream_t s);
+                   int n, void* chain, int32_t* ticket, float* out_acc, int interval, float target, float lo,
+                   float hi, hipStream_t s);
 
 }  // namespace dwmi

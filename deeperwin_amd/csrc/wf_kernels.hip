@@ -15,6 +15,14 @@ namespace dwmi {
 
 static constexpr int MAXL = 4;  // FermiNet iterations supported by the fused 2-el kernel
 
+// The device chain record (dw_chain_scalars).
+struct ChainScalars {
+  float stepsize;
+  float acc_rate;
+  int32_t step_nr;
+  int32_t pad;
+};
+
 // Scalar-type generic math (T = float for the FP32/BF16 forms, double for the FP64 form).
 __device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
 __device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
@@ -30,9 +38,12 @@ __device__ __forceinline__ double dexp(double x) { return exp(x); }
 // ------------------------------------------------------------------------------------------------
 template <typename T>
 __global__ void k_features0(const float* __restrict__ r, const float* __restrict__ R, T* __restrict__ X0, int Kx0,
-                            int tail0, int tailw, Shape sh) {
+                            int tail0, int tailw, Shape sh, const float* __restrict__ normals,
+                            const ChainScalars* __restrict__ chain, float* __restrict__ r_out) {
   // one thread per (electron row bi, column): the distance to the column's ion is formed once and the
-  // column is written for every channel row (consecutive threads write consecutive columns of a row)
+  // column is written for every channel row (consecutive threads write consecutive columns of a row).
+  // With normals, the positions are the Gaussian proposal r + stepsize * normals (k_propose's expression),
+  // stored to r_out by the thread of column 0 for the kernels after this one.
   const long total = (long)sh.B * sh.n * Kx0;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
@@ -41,6 +52,16 @@ __global__ void k_features0(const float* __restrict__ r, const float* __restrict
   const int i = (int)(bi - div_idx(bi, sh.n) * sh.n);
   const int I4 = 4 * sh.I, C = sh.C, Cp = sh.Cp;
   T* out = X0 + bi * Cp * (long)Kx0 + col;
+  float rp[3] = {0.f, 0.f, 0.f};
+  if (normals && (col == 0 || col < I4)) {
+    const float step = chain->stepsize;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) rp[c] = r[bi * 3 + c] + normals[bi * 3 + c] * step;
+    if (col == 0) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) r_out[bi * 3 + c] = rp[c];
+    }
+  }
   if (col >= I4) {
     // the columns [tail0, tail0 + tailw) are written by the 2-el kernels for the real channels; padding
     // columns and rows are zero
@@ -48,7 +69,7 @@ __global__ void k_features0(const float* __restrict__ r, const float* __restrict
     return;
   }
   const int ion = col >> 2, f = col & 3;
-  const float* ri = r + bi * 3;
+  const float* ri = normals ? rp : r + bi * 3;
   const T d[3] = {(T)ri[0] - (T)R[ion * 3 + 0], (T)ri[1] - (T)R[ion * 3 + 1], (T)ri[2] - (T)R[ion * 3 + 2]};
   const T rho = dsqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
   out[0] = (f == 0) ? rho : d[f - 1];
@@ -69,14 +90,15 @@ __global__ void k_features0(const float* __restrict__ r, const float* __restrict
 
 template <typename T>
 void launch_features0(const float* r, const float* R, T* X0, int Kx0, int tail0, int tailw, const Shape& sh,
-                      hipStream_t s) {
+                      hipStream_t s, const float* normals, const void* chain, float* r_out) {
   const long total = (long)sh.B * sh.n * Kx0;
   hipLaunchKernelGGL(k_features0<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, r, R, X0, Kx0, tail0,
-                     tailw, sh);
+                     tailw, sh, normals, (const ChainScalars*)chain, r_out);
 }
-template void launch_features0<float>(const float*, const float*, float*, int, int, int, const Shape&, hipStream_t);
-template void launch_features0<double>(const float*, const float*, double*, int, int, int, const Shape&,
-                                       hipStream_t);
+template void launch_features0<float>(const float*, const float*, float*, int, int, int, const Shape&, hipStream_t,
+                                      const float*, const void*, float*);
+template void launch_features0<double>(const float*, const float*, double*, int, int, int, const Shape&, hipStream_t,
+                                       const float*, const void*, float*);
 
 // ------------------------------------------------------------------------------------------------
 // Fused two-electron stream.  One workgroup per (walker b, group of 8 electrons i); 256 threads =
@@ -1749,12 +1771,6 @@ void launch_combine(const double* ld, const double* sd, const double* g, const d
 // ------------------------------------------------------------------------------------------------
 // Metropolis-Hastings pieces (mcmc.py:175-180, 345-387).
 // ------------------------------------------------------------------------------------------------
-struct ChainScalars {
-  float stepsize;
-  float acc_rate;
-  int32_t step_nr;
-  int32_t pad;
-};
 
 __global__ void k_fill_i32(int32_t* p, int32_t v, int count) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1799,12 +1815,17 @@ void launch_propose(const float* r, const float* normals, const void* chain, flo
 
 // accept = exp(lp' - lp + log q) > u  or  old age >= max_age;  age <- 0 / age + 1;  lp and r selected.
 // log q is the proposal's log density ratio (absent, i.e. 0, for the symmetric Gaussian proposal).
+__device__ __forceinline__ void chain_update(ChainScalars* chain, int accepted, int B, float* out_acc, int interval,
+                                             float target, float lo, float hi);
+
 __global__ __launch_bounds__(256) void k_accept(float* __restrict__ r, const float* __restrict__ r_prop,
                                                 float* __restrict__ lp, const float* __restrict__ lp_prop,
                                                 const float* __restrict__ logq,
                                                 const float* __restrict__ uniforms, int32_t* __restrict__ age,
                                                 int max_age, int32_t* __restrict__ acc_count,
-                                                uint32_t* __restrict__ flags, int B, int n) {
+                                                uint32_t* __restrict__ flags, int B, int n, ChainScalars* chain,
+                                                int32_t* __restrict__ ticket, float* out_acc, int interval,
+                                                float target, float lo, float hi) {
   // 64 walkers per workgroup: wave 0 decides (one lane per walker), all waves copy accepted positions
   __shared__ int acc_sh[64];
   const int tid = threadIdx.x;
@@ -1834,20 +1855,30 @@ __global__ __launch_bounds__(256) void k_accept(float* __restrict__ r, const flo
     const int w = (int)(t / per);
     if (acc_sh[w]) r[base + t] = r_prop[base + t];
   }
+  // the last workgroup to finish (a ticket per step, zeroed before the chain) updates the chain from the
+  // complete count; thread 0 made this workgroup's addition to the count above
+  if (tid == 0) {
+    __threadfence();
+    if (atomicAdd(ticket, 1) == (int)gridDim.x - 1) {
+      __threadfence();
+      chain_update(chain, atomicAdd(acc_count, 0), B, out_acc, interval, target, lo, hi);
+    }
+  }
 }
 void launch_accept(float* r, const float* r_prop, float* lp, const float* lp_prop, const float* logq,
                    const float* uniforms, int32_t* age, int max_age, int32_t* acc_count, uint32_t* flags, int B,
-                   int n, hipStream_t s) {
+                   int n, void* chain, int32_t* ticket, float* out_acc, int interval, float target, float lo,
+                   float hi, hipStream_t s) {
   hipLaunchKernelGGL(k_accept, dim3((unsigned)((B + 63) / 64)), dim3(256), 0, s, r, r_prop, lp, lp_prop, logq,
-                     uniforms, age, max_age, acc_count, flags, B, n);
+                     uniforms, age, max_age, acc_count, flags, B, n, (ChainScalars*)chain, ticket, out_acc,
+                     interval, target, lo, hi);
 }
 
 // acc = mean(accept); step_nr += 1; acc_rate = 0.9 acc_rate + 0.1 acc; every `interval` steps the
 // stepsize is scaled by 1/1.05 or 1.05 from the OLD acc_rate and clipped (mcmc.py:367-387).
-__global__ void k_chain_update(ChainScalars* chain, const int32_t* acc_count, int B, float* out_acc, int interval,
-                               float target, float lo, float hi) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const float acc = (float)(*acc_count) / (float)B;
+__device__ __forceinline__ void chain_update(ChainScalars* chain, int accepted, int B, float* out_acc, int interval,
+                                             float target, float lo, float hi) {
+  const float acc = (float)accepted / (float)B;
   const float old_rate = chain->acc_rate;
   const float old_step = chain->stepsize;
   chain->step_nr += 1;
@@ -1858,11 +1889,6 @@ __global__ void k_chain_update(ChainScalars* chain, const int32_t* acc_count, in
     chain->stepsize = s;
   }
   if (out_acc) *out_acc = acc;
-}
-void launch_chain_update(void* chain, const int32_t* acc_count, int B, float* out_acc, int interval, float target,
-                         float lo, float hi, hipStream_t s) {
-  hipLaunchKernelGGL(k_chain_update, dim3(1), dim3(64), 0, s, (ChainScalars*)chain, acc_count, B, out_acc, interval,
-                     target, lo, hi);
 }
 
 }  // namespace dwmi
