@@ -675,12 +675,15 @@ dw_status run_pass(dw_model* m, const PassBufs<T>& ws, const float* r, long B, c
   const OrbSrc<T> src{ws.O, m->NO, r, R, W.kappa.as<T>(), W.wenv.as<T>()};
   if (!det_reads_orbitals(m->n, C))
     launch_envelope<T>(ws.O, m->NO, r, R, W.kappa.as<T>(), W.wenv.as<T>(), ws.A, sh, s);
+  bool combined = false;
   {
     Timed t(m, C > 1 ? DW_TIMER_DET_LAP : -1, s);
-    launch_det<T>(ws.A, src, ws.ld, ws.sd, ws.g, ws.lap, ws.dcond, sh, s);
+    const CombineValue cv{out_lp, out_phase, out_flags, out_kappa, cond_thr, 0};
+    combined = launch_det<T>(ws.A, src, ws.ld, ws.sd, ws.g, ws.lap, ws.dcond, sh, s, C == 1 ? &cv : nullptr);
   }
-  launch_combine(ws.ld, ws.sd, ws.g, ws.lap, ws.dcond, cond_thr, r, R, Z, out_lp, out_phase, out_eloc, out_flags,
-                 out_kappa, sh, s);
+  if (!combined)
+    launch_combine(ws.ld, ws.sd, ws.g, ws.lap, ws.dcond, cond_thr, r, R, Z, out_lp, out_phase, out_eloc, out_flags,
+                   out_kappa, sh, s);
   HIP_TRY(hipGetLastError());
   return DW_OK;
 }

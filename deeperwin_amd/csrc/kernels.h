@@ -96,9 +96,20 @@ struct OrbSrc {
 // True when the determinant kernels for n electrons and C channels apply the envelope while reading O
 // (no envelope launch, no stored product); otherwise launch_envelope fills A first.
 inline bool det_reads_orbitals(int n, int C) { return C == 1 ? n <= 32 : n <= 16; }
+// Outputs of the value-pass combine (launch_combine's, C == 1) when launch_det runs it itself.
+struct CombineValue {
+  float* out_lp;
+  float* out_phase;
+  uint32_t* out_flags;
+  float* out_kappa;
+  float cond_thr;
+  int fuse;  // set by launch_det
+};
+// Returns true when it also ran the value-pass combine for `combine` (value pass, n <= 32, D dividing the
+// determinants per workgroup); the caller then skips launch_combine.
 template <typename TA>
-void launch_det(const TA* A, const OrbSrc<TA>& src, double* ld, double* sd, double* g, double* lapd, float* dcond,
-                const Shape& sh, hipStream_t s);
+bool launch_det(const TA* A, const OrbSrc<TA>& src, double* ld, double* sd, double* g, double* lapd, float* dcond,
+                const Shape& sh, hipStream_t s, const CombineValue* combine = nullptr);
 void launch_combine(const double* ld, const double* sd, const double* g, const double* lapd, const float* dcond,
                     float cond_thr, const float* r, const float* R, const float* Z, float* out_lp, float* out_phase,
                     float* out_eloc, uint32_t* out_flags, float* out_kappa, const Shape& sh, hipStream_t s);

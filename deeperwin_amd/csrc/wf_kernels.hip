@@ -931,6 +931,35 @@ __device__ __forceinline__ void finish_slogdet(double mydiag, int swaps, bool si
   }
 }
 
+// Value-pass combine of one walker's D determinants (log|det| l, sign sg, matrix conditioning dc).
+__device__ __forceinline__ void combine_value_walker(const double* l, const double* sg, const float* dc, int D,
+                                                     float cond_thr, long b, float* __restrict__ out_lp,
+                                                     float* __restrict__ out_phase, uint32_t* __restrict__ out_flags,
+                                                     float* __restrict__ out_kappa) {
+  double m = -INFINITY;
+  for (int d = 0; d < D; ++d) m = fmax(m, (double)l[d]);
+  double S = 0.0, Sabs = 0.0, cmax = 0.0;
+  uint32_t fl = 0;
+  for (int d = 0; d < D; ++d) {
+    const double e = sg[d] * exp(l[d] - m);
+    S += e;
+    Sabs += fabs(e);
+    cmax = fmax(cmax, (double)dc[d]);
+  }
+  // conditioning of log|psi|: matrix condition times the cancellation of the determinant sum
+  const double kappa = cmax * Sabs / fabs(S);
+  if (out_kappa) out_kappa[b] = (float)kappa;
+  if (cond_thr > 0.f && !(kappa <= (double)cond_thr)) fl |= DWF_FP64;
+  const double eps = 1e-8;
+  const double aS = fabs(S);
+  const double fval = log(aS + eps) + m;
+  const float lp = (float)(2.0 * fval);
+  if (out_lp) out_lp[b] = lp;
+  if (out_phase) out_phase[b] = (S < 0.0) ? 3.14159265358979323846f : 0.f;
+  if (!isfinite(lp)) fl |= DWF_NONFINITE;
+  if (out_flags) out_flags[b] = fl;
+}
+
 // Value mode: 64 / NMAX matrices per wavefront (lane group g holds matrix g), 4 wavefronts per workgroup.
 // The matrix is formed with a lane per column (coalesced reads of O), staged in wave-private LDS, and
 // factorised with a lane per row: a row interchange is then a relabelling (no data moves), the pivot is a
@@ -941,7 +970,8 @@ __device__ __forceinline__ void finish_slogdet(double mydiag, int swaps, bool si
 // too; the sign counts the negative pivots and the parity of the pivot order.
 template <int NMAX, typename TA>
 __global__ __launch_bounds__(256) void k_det_value(OrbSrc<TA> src, double* __restrict__ ld, double* __restrict__ sd,
-                                                   float* __restrict__ dcond, long nmat, int n, Shape sh) {
+                                                   float* __restrict__ dcond, long nmat, int n, Shape sh,
+                                                   CombineValue co) {
   constexpr int G = 64 / NMAX;
   const int lane = threadIdx.x & 63;
   const int g = lane / NMAX, t = lane % NMAX, base = g * NMAX;
@@ -1026,10 +1056,33 @@ __global__ __launch_bounds__(256) void k_det_value(OrbSrc<TA> src, double* __res
     umax = fmax(umax, __shfl_xor(umax, o, NMAX));
     umin = fmin(umin, __shfl_xor(umin, o, NMAX));
   }
+  const double ldv = sing ? -INFINITY : lg, sdv = sing ? 0.0 : (((neg + inv) & 1) ? -1.0 : 1.0);
+  const float dcv = (float)(umax / umin);
   if (t == 0 && valid) {
-    ld[bd] = sing ? -INFINITY : lg;
-    sd[bd] = sing ? 0.0 : (((neg + inv) & 1) ? -1.0 : 1.0);
-    dcond[bd] = (float)(umax / umin);
+    ld[bd] = ldv;
+    sd[bd] = sdv;
+    dcond[bd] = dcv;
+  }
+  if (co.fuse) {
+    // the workgroup's 256 / NMAX matrices are whole walkers (D divides that number): combine them here
+    constexpr int MPW = 256 / NMAX;
+    __shared__ double c_ld[MPW], c_sd[MPW];
+    __shared__ float c_dc[MPW];
+    const int mi = (int)(threadIdx.x >> 6) * G + g;
+    if (t == 0) {
+      c_ld[mi] = ldv;
+      c_sd[mi] = sdv;
+      c_dc[mi] = dcv;
+    }
+    __syncthreads();
+    const int D = sh.D, wpw = MPW / D;
+    if ((int)threadIdx.x < wpw) {
+      const long b = (long)blockIdx.x * wpw + threadIdx.x;
+      const int o = (int)threadIdx.x * D;
+      if (b < sh.B)
+        combine_value_walker(c_ld + o, c_sd + o, c_dc + o, D, co.cond_thr, b, co.out_lp, co.out_phase, co.out_flags,
+                             co.out_kappa);
+    }
   }
 }
 
@@ -1565,8 +1618,8 @@ __global__ __launch_bounds__(64) void k_det_value_lds(const TA* __restrict__ A, 
 }
 
 template <typename TA>
-void launch_det(const TA* A, const OrbSrc<TA>& src, double* ld, double* sd, double* g, double* lapd, float* dcond,
-                const Shape& sh, hipStream_t s) {
+bool launch_det(const TA* A, const OrbSrc<TA>& src, double* ld, double* sd, double* g, double* lapd, float* dcond,
+                const Shape& sh, hipStream_t s, const CombineValue* combine) {
   const int n = sh.n;
   const long nmat = (long)sh.B * sh.D;
   const long stride = (long)sh.C * n * n;
@@ -1581,20 +1634,25 @@ void launch_det(const TA* A, const OrbSrc<TA>& src, double* ld, double* sd, doub
       hipLaunchKernelGGL((k_det_lds<TA>), dim3((unsigned)nmat), dim3(256), shm, s, A, ld, sd, g, lapd, dcond, n,
                          sh.C);
     }
-    return;
+    return false;
   }
   if (sh.C == 1) {
-    // 4 wavefronts per workgroup, 64 / NMAX matrices per wavefront; the envelope is applied as O is read
+    // 4 wavefronts per workgroup, 64 / NMAX matrices per wavefront; the envelope is applied as O is read.
+    // When D divides the matrices per workgroup, the walkers' combine runs at the end of the same kernel.
+    const int mpw = n <= 8 ? 32 : (n <= 16 ? 16 : 8);
+    CombineValue co{};
+    if (combine && mpw % sh.D == 0) {
+      co = *combine;
+      co.fuse = 1;
+    }
+    const dim3 grid((unsigned)((nmat + mpw - 1) / mpw));
     if (n <= 8)
-      hipLaunchKernelGGL((k_det_value<8, TA>), dim3((unsigned)((nmat + 31) / 32)), dim3(256), 0, s, src, ld, sd,
-                         dcond, nmat, n, sh);
+      hipLaunchKernelGGL((k_det_value<8, TA>), grid, dim3(256), 0, s, src, ld, sd, dcond, nmat, n, sh, co);
     else if (n <= 16)
-      hipLaunchKernelGGL((k_det_value<16, TA>), dim3((unsigned)((nmat + 15) / 16)), dim3(256), 0, s, src, ld, sd,
-                         dcond, nmat, n, sh);
+      hipLaunchKernelGGL((k_det_value<16, TA>), grid, dim3(256), 0, s, src, ld, sd, dcond, nmat, n, sh, co);
     else
-      hipLaunchKernelGGL((k_det_value<32, TA>), dim3((unsigned)((nmat + 7) / 8)), dim3(256), 0, s, src, ld, sd,
-                         dcond, nmat, n, sh);
-    return;
+      hipLaunchKernelGGL((k_det_value<32, TA>), grid, dim3(256), 0, s, src, ld, sd, dcond, nmat, n, sh, co);
+    return co.fuse != 0;
   }
   if (n <= 16) {
     // M^-1 (FP64) plus the envelope tables (5 [16][16] arrays of TA) per wavefront
@@ -1608,11 +1666,12 @@ void launch_det(const TA* A, const OrbSrc<TA>& src, double* ld, double* sd, doub
     hipLaunchKernelGGL((k_det_lap<NM, W, TA, false>), dim3((unsigned)((nmat + W - 1) / W)), dim3(64 * W), shm, s, A,
                        src, sh, ld, sd, g, lapd, dcond, nmat, n, sh.C);
   }
+  return false;
 }
-template void launch_det<float>(const float*, const OrbSrc<float>&, double*, double*, double*, double*, float*,
-                                const Shape&, hipStream_t);
-template void launch_det<double>(const double*, const OrbSrc<double>&, double*, double*, double*, double*, float*,
-                                 const Shape&, hipStream_t);
+template bool launch_det<float>(const float*, const OrbSrc<float>&, double*, double*, double*, double*, float*,
+                                const Shape&, hipStream_t, const CombineValue*);
+template bool launch_det<double>(const double*, const OrbSrc<double>&, double*, double*, double*, double*, float*,
+                                 const Shape&, hipStream_t, const CombineValue*);
 
 // ------------------------------------------------------------------------------------------------
 // Per-walker combination (FP64).  Sum of determinants with max shift and LOG_EPSILON
@@ -1632,33 +1691,8 @@ __global__ void k_combine_value(const double* __restrict__ ld, const double* __r
                           uint32_t* __restrict__ out_flags, float* __restrict__ out_kappa, Shape sh) {
   const long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= sh.B) return;
-  const int D = sh.D, n = sh.n, nd = 3 * n;
-  const double* l = ld + b * D;
-  const double* sg = sd + b * D;
-  double m = -INFINITY;
-  for (int d = 0; d < D; ++d) m = fmax(m, (double)l[d]);
-  double S = 0.0, Sabs = 0.0, cmax = 0.0;
-  int cnt = 0;
-  uint32_t fl = 0;
-  for (int d = 0; d < D; ++d) {
-    const double e = sg[d] * exp(l[d] - m);
-    S += e;
-    Sabs += fabs(e);
-    if (l[d] == m) ++cnt;
-    cmax = fmax(cmax, (double)dcond[b * D + d]);
-  }
-  // conditioning of log|psi|: matrix condition times the cancellation of the determinant sum
-  const double kappa = cmax * Sabs / fabs(S);
-  if (out_kappa) out_kappa[b] = (float)kappa;
-  if (cond_thr > 0.f && !(kappa <= (double)cond_thr)) fl |= DWF_FP64;
-  const double eps = 1e-8;
-  const double aS = fabs(S);
-  const double fval = log(aS + eps) + m;
-  const float lp = (float)(2.0 * fval);
-  if (out_lp) out_lp[b] = lp;
-  if (out_phase) out_phase[b] = (S < 0.0) ? 3.14159265358979323846f : 0.f;
-  if (!isfinite(lp)) fl |= DWF_NONFINITE;
-  if (out_flags) out_flags[b] = fl;
+  const int D = sh.D;
+  combine_value_walker(ld + b * D, sd + b * D, dcond + b * D, D, cond_thr, b, out_lp, out_phase, out_flags, out_kappa);
 }
 
 // Laplacian pass: one wavefront per walker; the formulas above.  Lanes split the 3n gradient components
