@@ -669,7 +669,7 @@ def main():
         tuning, _ = autotune_model(model, r, Rd, Zd, reps=3)
 
     # proposal normals and accept uniforms live in fixed buffers, refilled in place before every unit, so
-    # the S MCMC steps (one dw_mcmc_steps call, ~21 launches per step) can be replayed as one HIP graph
+    # the S MCMC steps (one dw_mcmc_steps call, ~18 launches per step) can be replayed as one HIP graph
     normals = torch.empty((S, B, n, 3), device=dev)
     uniforms = torch.empty((S, B), device=dev)
 
