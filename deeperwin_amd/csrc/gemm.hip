@@ -832,16 +832,17 @@ void gemm_value16(const float* X, int ldx, const float* Bt, int K, const float* 
 // Per-walker term of a 1-el layer in the Laplacian pass (FP32), spin means fused in:
 //   V[r, o] = sum_k MU[r, k] Wmu[o, k] (+ bias[o] on each walker's value channel),
 //   MU[r, k] = mean over the electrons of spin s(k) of X[(b n + i) Cp + ch, k - s Kh],  r = b Cp + ch,
-// with MU formed in LDS for 48 rows and 256 of its columns at a time (never stored).  The means add the
+// with MU formed in LDS for 48 rows and 128 of its columns at a time (never stored; 25 KB of LDS and 124
+// registers, so four workgroups per CU overlap one another's load and MFMA phases: 522 -> 478 us per N2 layer).  The means add the
 // electrons in order and scale by 1 / n_s like k_spin_mean, and the MFMA sequence per output element
 // (16-wide K slices in order, step s of lane group g taking k = 4 g + s) is k_gemm_f32's, so V equals
 // what those two kernels produce.  4 waves x 64 output columns (N = 256); each wave's weights come
-// straight from global memory (L2), one slice ahead.
+// straight from global memory (L2).
 // ------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_mean_gemm(const float* __restrict__ X, int Kx, int Kh,
                                                    const float* __restrict__ Bt, int Km, const float* __restrict__ bias,
                                                    float* __restrict__ V, int ldv, Shape sh) {
-  constexpr int RT = 48, RB = RT / 16, CW = 256, PITCH = CW + 4;
+  constexpr int RT = 48, RB = RT / 16, CW = 128, PITCH = CW + 4;
   __shared__ __align__(16) float Ms[RT][PITCH];
   const long rows = (long)sh.B * sh.Cp;
   const long r0 = (long)blockIdx.x * RT;
@@ -888,14 +889,13 @@ __global__ __launch_bounds__(256) void k_mean_gemm(const float* __restrict__ X, 
       *(v4f*)&Ms[rr][f] = v4f{v0, v1, v2, v3};
     }
     __syncthreads();
-    v4f bq[4], bn[4];
+    v4f bq[4];
     auto bload = [&](int k0, v4f* b) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) b[q] = *(const v4f*)(Bt + (long)(cw + 16 * q + cl) * Km + kc + k0 + 4 * g);
     };
-    bload(0, bq);
     for (int k0 = 0; k0 < cwid; k0 += 16) {
-      if (k0 + 16 < cwid) bload(k0 + 16, bn);
+      bload(k0, bq);
       v4f a4[RB];
 #pragma unroll
       for (int rb = 0; rb < RB; ++rb) a4[rb] = *(const v4f*)&Ms[rb * 16 + cl][k0 + 4 * g];
@@ -906,10 +906,6 @@ __global__ __launch_bounds__(256) void k_mean_gemm(const float* __restrict__ X, 
 #pragma unroll
           for (int q = 0; q < 4; ++q)
             acc[rb][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[rb][st], bq[q][st], acc[rb][q], 0, 0, 0);
-      if (k0 + 16 < cwid) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) bq[q] = bn[q];
-      }
     }
   }
 #pragma unroll
