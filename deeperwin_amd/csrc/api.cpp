@@ -1198,7 +1198,7 @@ dw_status dw_model_set_tuning(dw_model* m, const dw_tuning* t) {
   if (!m || !t) return fail(DW_ERR_INVALID, "null argument");
   for (int32_t v : {t->fwd_orb_tile, t->lap_orb_tile, t->lap_mu_tile})
     if (v < -1 || v > 2) return fail(DW_ERR_INVALID, "tile must be -1, 0, 1 or 2");
-  if (t->fwd_main_tile < -1 || t->fwd_main_tile > 3) return fail(DW_ERR_INVALID, "fwd_main_tile must be -1 .. 3");
+  if (t->fwd_main_tile < -1 || t->fwd_main_tile > 4) return fail(DW_ERR_INVALID, "fwd_main_tile must be -1 .. 4");
   if (!(t->fwd_mu_split == 0 || t->fwd_mu_split == 1 || t->fwd_mu_split == 2 || t->fwd_mu_split == 4))
     return fail(DW_ERR_INVALID, "fwd_mu_split must be 0 (auto), 1, 2 or 4");
   const int Cp = rows_per_electron(3 * m->n + 2);

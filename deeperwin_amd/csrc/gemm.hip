@@ -387,6 +387,145 @@ static void launch_f32(const float* A, int lda, const float* Bt, int ldb, float*
                        K, tilesN, total, ep);
 }
 
+// ------------------------------------------------------------------------------------------------
+// FP32 MFMA on full-width row tiles: 112 rows x all 256 columns per 512-thread workgroup (the A tile is
+// read once; at 28,672 rows, one workgroup per CU), wave w owning columns [32 w, 32 w + 32) of all rows
+// (7 x 2 blocks of 16 x 16, 14 MFMAs per step against 9 operand reads; two waves per SIMD).  Both operand slices are
+// double-buffered in LDS (one barrier per 16-wide slice) with the next slice's loads in registers.  The
+// slices, the LDS layout and the MFMA step order (step s of lane group g takes k = 4 g + s) are
+// k_gemm_f32's, and so is the epilogue: results are bit-identical to it.  N = 256, K % 16 == 0, no split-K;
+// with a V epilogue, C == 1 and n >= 8.
+// ------------------------------------------------------------------------------------------------
+template <int FLAGS>
+__global__ __launch_bounds__(512) void k_gemm_rows256(const float* __restrict__ A, int lda,
+                                                      const float* __restrict__ Bt, int ldb, float* __restrict__ Cm,
+                                                      int ldc, int M, int K, int tilesTotal, GemmEpi ep) {
+  constexpr int NT = 512;  // 8 waves, 2 per SIMD
+  constexpr int RB = 7, BM = 16 * RB, BN = 256, BK = 16, PITCH = BK + 4, KV = BK / 4, NQ = 2;
+  constexpr int AV = BM * KV, BV = BN * KV, AIT = (AV + NT - 1) / NT, BIT = BV / NT;
+  __shared__ __align__(16) float As[2][BM][PITCH];
+  __shared__ __align__(16) float Bs[2][BN][PITCH];
+  const int tile = tile_index(blockIdx.x, tilesTotal);
+  if (tile >= tilesTotal) return;  // uniform over the workgroup, before any barrier
+  const long row0 = (long)tile * BM;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int g = lane >> 4, cl = lane & 15;
+
+  v4f acc[RB][NQ];
+#pragma unroll
+  for (int a = 0; a < RB; ++a)
+#pragma unroll
+    for (int b = 0; b < NQ; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+
+  auto srow = [](int idx) { return (idx / (16 * KV)) * 16 + (idx & 15); };
+  auto squad = [](int idx) { return ((idx >> 4) % KV) * 4; };
+  v4f ra[AIT], rb[BIT];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int it = 0; it < AIT; ++it) {
+      const int idx = tid + NT * it;
+      if (idx < AV) {
+        long gr = row0 + srow(idx);
+        if (gr > M - 1) gr = M - 1;
+        ra[it] = *(const v4f*)(A + gr * lda + k0 + squad(idx));
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < BIT; ++it) {
+      const int idx = tid + NT * it;
+      rb[it] = *(const v4f*)(Bt + (long)srow(idx) * ldb + k0 + squad(idx));
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int it = 0; it < AIT; ++it) {
+      const int idx = tid + NT * it;
+      if (idx < AV) *(v4f*)&As[buf][srow(idx)][squad(idx)] = ra[it];
+    }
+#pragma unroll
+    for (int it = 0; it < BIT; ++it) {
+      const int idx = tid + NT * it;
+      *(v4f*)&Bs[buf][srow(idx)][squad(idx)] = rb[it];
+    }
+  };
+
+  const int S = K / BK;
+  gload(0);
+  sstore(0);
+  if (S > 1) gload(BK);
+  __syncthreads();
+  for (int sl = 0; sl < S; ++sl) {
+    const int buf = sl & 1;
+    v4f a4[RB], b4[NQ];
+#pragma unroll
+    for (int i = 0; i < RB; ++i) a4[i] = *(const v4f*)&As[buf][i * 16 + cl][4 * g];
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) b4[j] = *(const v4f*)&Bs[buf][wid * 16 * NQ + j * 16 + cl][4 * g];
+#pragma unroll
+    for (int st = 0; st < 4; ++st)
+#pragma unroll
+      for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < NQ; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[i][st], b4[j][st], acc[i][j], 0, 0, 0);
+    // the other buffer was last read before the previous barrier; the next slice's compute reads it
+    // after this one
+    if (sl + 1 < S) {
+      sstore(buf ^ 1);
+      if (sl + 2 < S) gload((sl + 2) * BK);
+    }
+    __syncthreads();
+  }
+
+  // Epilogue.  With V (the launcher guarantees one channel per electron and n >= 8, so a tile spans at most
+  // 15 walkers) the walkers' V sums (parts added in epi_store's order) are formed once in LDS after the
+  // main loop's last barrier, and the 112 outputs of a lane read no V from memory.
+  constexpr int WMAX = 16;
+  static_assert(WMAX * BN <= 2 * BM * PITCH, "V staging fits in the A buffers");
+  float(*Vs)[BN] = (float(*)[BN]) & As[0][0][0];
+  int b_first = 0;
+  if constexpr ((FLAGS & EPI_V) != 0) {
+    const long r_last = (row0 + BM < M ? row0 + BM : (long)M) - 1;
+    b_first = (int)(row0 / ep.n);
+    const int nbw = (int)(r_last / ep.n) - b_first + 1;
+    for (int idx = tid; idx < nbw * BN; idx += NT) {
+      const int w = idx / BN, c = idx - w * BN;
+      const float* Vp = ep.V + (long)(b_first + w) * ep.ldv + c;
+      float vs = Vp[0];
+      for (int q = 1; q < ep.vsplit; ++q) vs += Vp[q * ep.vstride];
+      Vs[w][c] = vs;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int j = 0; j < NQ; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = (int)row0 + i * 16 + 4 * g + e;
+        const int c = wid * 16 * NQ + j * 16 + cl;
+        if (r < M) {
+          if constexpr ((FLAGS & EPI_V) != 0) {
+            // epi_store with C == 1: bias on every row, then V, tanh, spin-major permutation
+            float v = acc[i][j][e];
+            if ((FLAGS & EPI_BIAS) && ep.bias) v += ep.bias[c];
+            const int b = r / ep.n;
+            v += Vs[b - b_first][c];
+            if ((FLAGS & EPI_TANH) && ep.act_tanh) v = act_tanh(v);
+            long orow = r;
+            if ((FLAGS & EPI_PERM) && ep.permute) {
+              const int el = r - b * ep.n, dn = ep.n - ep.u;
+              orow = (el < ep.u) ? (long)b * ep.u + el : (long)ep.B * ep.u + ((long)b * dn + (el - ep.u));
+            }
+            Cm[orow * ldc + c] = v;
+          } else {
+            epi_store<FLAGS, float>(acc[i][j][e], r, c, Cm, ldc, ep);
+          }
+        }
+      }
+}
+
 // Workgroups needed to fill the chip a few times over: the largest tile that still gives >= 4 per CU.
 static int pick_tile(long M, int N) {
   const long wg128 = ((M + 127) / 128) * ((N + 127) / 128);
@@ -401,8 +540,23 @@ static void gemm_dispatch(GemmPrec prec, const float* A, int lda, const float* B
                           const uint16_t* Blo, int ldb, float* C, int ldc, int M, int N, int K, const GemmEpi& ep,
                           hipStream_t s) {
   if (prec == GEMM_F32) {
-    // Bt holds pad128(N) rows, so every tile shape reads inside it
-    switch (ep.tile >= 0 ? ep.tile : pick_tile((long)M * ep.ksplit, N)) {
+    // Bt holds pad128(N) rows, so every tile shape reads inside it.  Tile 4 (112 x 256 full-width rows,
+    // N = 256 only) is what the heuristic takes where it would pick 64 x 64 tiles.
+    int tile = ep.tile >= 0 ? ep.tile : pick_tile((long)M * ep.ksplit, N);
+    if (ep.tile < 0 && tile == 0) tile = 4;
+    if (tile == 4 && !(N == 256 && ep.ksplit == 1 && K % 16 == 0 && K >= 16 && (!ep.V || (ep.C == 1 && ep.n >= 8))))
+      tile = 0;
+    if (tile == 4) {
+      // (not for the generic all-flags instance, whose epilogue would not stay in registers)
+      if constexpr (FLAGS != (EPI_BIAS | EPI_V | EPI_TANH | EPI_PERM)) {
+        const int total = (M + 111) / 112;
+        hipLaunchKernelGGL((k_gemm_rows256<FLAGS>), dim3((total + 7) / 8 * 8), dim3(512), 0, s, A, lda, Bt, ldb, C,
+                           ldc, M, K, total, ep);
+        return;
+      }
+      tile = 0;
+    }
+    switch (tile) {
       case 2: launch_f32<FLAGS, 2, 2>(A, lda, Bt, ldb, C, ldc, M, N, K, ep, s); break;
       case 1: launch_f32<FLAGS, 1, 2>(A, lda, Bt, ldb, C, ldc, M, N, K, ep, s); break;
       default: launch_f32<FLAGS, 1, 1>(A, lda, Bt, ldb, C, ldc, M, N, K, ep, s); break;

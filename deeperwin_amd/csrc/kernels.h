@@ -18,7 +18,8 @@ struct GemmEpiT {
   int n = 0, u = 0, B = 0;      // shape for `permute`
   int vsplit = 1;               // V is the sum of vsplit partial buffers V + s * vstride
   long vstride = 0;
-  int tile = -1;                // FP32 tile: -1 = heuristic, 0 = 64x64, 1 = 64x128, 2 = 128x128
+  int tile = -1;                // FP32 tile: -1 = heuristic, 0 = 64x64, 1 = 64x128, 2 = 128x128,
+                                // 4 = 112x256 (N = 256, no split-K; else 64x64)
   int ksplit = 1;               // split K over ksplit partial outputs C + s * cstride (bias on s = 0 only)
   long cstride = 0;
 };

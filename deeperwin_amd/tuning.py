@@ -44,7 +44,7 @@ def lap_electron_choices(n_el: int) -> List[int]:
 def search_space(n_el: int) -> List[Tuple[str, Sequence[int], str]]:
     """(field, candidates, pass it affects) in the order the sweep visits them."""
     return [
-        ("fwd_main_tile", TILE_CHOICES + (3,), "value"),
+        ("fwd_main_tile", TILE_CHOICES + (3, 4), "value"),
         ("fwd_mu_split", SPLIT_CHOICES, "value"),
         ("fwd_orb_tile", TILE_CHOICES, "value"),
         ("lap_cols", (128, 256), "lap"),

@@ -113,7 +113,8 @@ def test_tuning_changes_launch_shapes_only(cuda):
     lp0, e0 = run()
     # GEMM tiles: every output element sums the same products in the same order
     for choice in ({"fwd_main_tile": 0, "fwd_orb_tile": 2, "lap_orb_tile": 0, "lap_mu_tile": 2},
-                   {"fwd_main_tile": 2, "fwd_orb_tile": 1, "lap_orb_tile": 1, "lap_mu_tile": 0}):
+                   {"fwd_main_tile": 2, "fwd_orb_tile": 1, "lap_orb_tile": 1, "lap_mu_tile": 0},
+                   {"fwd_main_tile": 4, "fwd_orb_tile": 0, "lap_orb_tile": 2, "lap_mu_tile": 1}):
         m.set_tuning(**choice)
         lp1, e1 = run()
         np.testing.assert_array_equal(lp1, lp0)
