@@ -47,11 +47,12 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--workload", default="n2", choices=["n2", "c4h2-precision", "serving", "batch64", "copy", "grad",
-                                                          "grad-dpe4", "dpe4"],
+                                                          "grad-dpe4", "dpe4", "vendor-gemm"],
                     help="n2: the headline line; c4h2-precision: the precision-form study (E_loc only); "
                          "serving: Poisson request stream with continuous batching; batch64: 64 geometries "
                          "under shared weights in one graph launch per unit; copy: a plain device copy "
-                         "of known size (calibrates the HBM byte counters)")
+                         "of known size (calibrates the HBM byte counters); vendor-gemm: the vendor FP32 GEMM "
+                         "(torch.matmul) on the N2 path's GEMM shapes, a reference point for the MFMA fraction")
     ap.add_argument("--requests", type=int, default=24, help="serving: requests in the stream")
     ap.add_argument("--rate", type=float, default=50.0, help="serving: offered load, requests/s")
     ap.add_argument("--fp64-threshold", type=float, default=1e6)
@@ -175,6 +176,43 @@ def copy_calibration(args):
     torch.cuda.synchronize()
     print(json.dumps({"workload": "copy", "bytes_read_per_launch": x.numel() * 4,
                       "bytes_written_per_launch": y.numel() * 4, "launches": 5}))
+
+
+def vendor_gemm_calibration(args):
+    """The vendor library's FP32 GEMM (torch.matmul, no TF32) on the GEMM shapes of the N2 workload (2048
+    walkers, 14 electrons, 7 up, 48 channel rows per electron in the Laplacian pass, widths 256/32, 16 full
+    dets): what a tuned plain GEMM reaches on these shapes, without the fused epilogues (V sums, tanh rule)
+    of this library's kernels.  Rows are the padded ones the kernels run."""
+    import torch
+
+    torch.backends.cuda.matmul.allow_tf32 = False
+    B, n, nu, Cp, W1, W2, D = 2048, 14, 7, 48, 256, 32, 16
+    shapes = [("laplacian 1-el GEMM", B * n * Cp, W1 + 2 * W2, W1),
+              ("laplacian orbital GEMM (one spin)", B * nu * Cp, W1, D * n),
+              ("value 1-el GEMM", B * n, W1 + 2 * W2, W1),
+              ("value orbital GEMM (one spin)", B * nu, W1, D * n)]
+    rows = []
+    g = torch.Generator(device="cuda").manual_seed(0)
+    for name, M, K, N in shapes:
+        A = torch.randn(M, K, device="cuda", generator=g)
+        Bt = torch.randn(N, K, device="cuda", generator=g)
+        C = torch.empty(M, N, device="cuda")
+        for _ in range(3):
+            torch.matmul(A, Bt.t(), out=C)
+        torch.cuda.synchronize()
+        reps = 20
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            torch.matmul(A, Bt.t(), out=C)
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) / reps * 1e3
+        tf = 2.0 * M * K * N / us / 1e6
+        rows.append({"gemm": name, "M": M, "K": K, "N": N, "us": round(us, 1), "tflops": round(tf, 2),
+                     "frac_fp32_peak": round(tf / PEAK_TFLOPS["fp32"], 3)})
+        del A, Bt, C
+    print(json.dumps({"workload": "vendor-gemm", "library": "torch.matmul (FP32, TF32 off)", "gemms": rows}))
 
 
 def grad_bench(args):
@@ -619,6 +657,9 @@ def main():
         return
     if args.workload == "copy":
         copy_calibration(args)
+        return
+    if args.workload == "vendor-gemm":
+        vendor_gemm_calibration(args)
         return
     if args.workload in ("grad", "grad-dpe4"):
         grad_bench(args)

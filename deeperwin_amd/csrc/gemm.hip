@@ -388,20 +388,21 @@ static void launch_f32(const float* A, int lda, const float* Bt, int ldb, float*
 }
 
 // ------------------------------------------------------------------------------------------------
-// FP32 MFMA on full-width row tiles: 112 rows x all 256 columns per 512-thread workgroup (the A tile is
-// read once; at 28,672 rows, one workgroup per CU), wave w owning columns [32 w, 32 w + 32) of all rows
-// (7 x 2 blocks of 16 x 16, 14 MFMAs per step against 9 operand reads; two waves per SIMD).  Both operand slices are
+// FP32 MFMA on full-width row tiles: 112 rows x all 256 columns per 1024-thread workgroup (the A tile is
+// read once; at 28,672 rows, one workgroup per CU), wave w owning columns [16 w, 16 w + 16) of all rows
+// (7 blocks of 16 x 16; four waves per SIMD hide each other's LDS and barrier waits: measured faster than
+// 8 waves of 7 x 2 blocks and 4 waves of 7 x 4 blocks).  Both operand slices are
 // double-buffered in LDS (one barrier per 16-wide slice) with the next slice's loads in registers.  The
 // slices, the LDS layout and the MFMA step order (step s of lane group g takes k = 4 g + s) are
 // k_gemm_f32's, and so is the epilogue: results are bit-identical to it.  N = 256, K % 16 == 0, no split-K;
 // with a V epilogue, C == 1 and n >= 8.
 // ------------------------------------------------------------------------------------------------
 template <int FLAGS>
-__global__ __launch_bounds__(512) void k_gemm_rows256(const float* __restrict__ A, int lda,
+__global__ __launch_bounds__(1024) void k_gemm_rows256(const float* __restrict__ A, int lda,
                                                       const float* __restrict__ Bt, int ldb, float* __restrict__ Cm,
                                                       int ldc, int M, int K, int tilesTotal, GemmEpi ep) {
-  constexpr int NT = 512;  // 8 waves, 2 per SIMD
-  constexpr int RB = 7, BM = 16 * RB, BN = 256, BK = 16, PITCH = BK + 4, KV = BK / 4, NQ = 2;
+  constexpr int NT = 1024;  // 16 waves, 4 per SIMD
+  constexpr int RB = 7, BM = 16 * RB, BN = 256, BK = 16, PITCH = BK + 4, KV = BK / 4, NQ = 1;
   constexpr int AV = BM * KV, BV = BN * KV, AIT = (AV + NT - 1) / NT, BIT = BV / NT;
   __shared__ __align__(16) float As[2][BM][PITCH];
   __shared__ __align__(16) float Bs[2][BN][PITCH];
@@ -550,7 +551,7 @@ static void gemm_dispatch(GemmPrec prec, const float* A, int lda, const float* B
       // (not for the generic all-flags instance, whose epilogue would not stay in registers)
       if constexpr (FLAGS != (EPI_BIAS | EPI_V | EPI_TANH | EPI_PERM)) {
         const int total = (M + 111) / 112;
-        hipLaunchKernelGGL((k_gemm_rows256<FLAGS>), dim3((total + 7) / 8 * 8), dim3(512), 0, s, A, lda, Bt, ldb, C,
+        hipLaunchKernelGGL((k_gemm_rows256<FLAGS>), dim3((total + 7) / 8 * 8), dim3(1024), 0, s, A, lda, Bt, ldb, C,
                            ldc, M, K, total, ep);
         return;
       }
