@@ -137,6 +137,45 @@ def test_tuning_changes_launch_shapes_only(cuda):
     assert np.all(np.abs(e3 - e0) <= tol_e)
 
 
+def test_row_tile_value_gemm_matches_small_tiles(cuda):
+    """The full-width row-tile value GEMM (the heuristic's choice at 14 electrons: V sums staged per
+    workgroup, 112-row tiles, here with a ragged last tile: 500 walkers = 7000 rows) sums the same
+    products in the same order as the 64 x 64 tiles: log|psi|, phase and E_loc are bit-equal."""
+    from deeperwin_amd._native import NativeModel
+    from deeperwin_amd.params import init_params
+    from deeperwin_amd.physics import generate_el_ion_mapping, geometry_n2, init_walkers_gaussian
+
+    R, Z = geometry_n2()
+    shape = ModelShape(n_el=14, n_up=7, n_ion=2, n_det=4)
+    params = init_params(shape, 3)
+    B = 500
+    r0 = init_walkers_gaussian(np.random.default_rng(4), B, R, generate_el_ion_mapping(R, Z, 14, 7))
+    r = torch.as_tensor(r0, dtype=torch.float32, device="cuda").contiguous()
+    Rd = torch.as_tensor(R, dtype=torch.float32, device="cuda")
+    Zd = torch.as_tensor(Z, dtype=torch.float32, device="cuda")
+    m = NativeModel(shape)
+    m.set_params(params)
+
+    def run():
+        lp = torch.empty(B, device="cuda")
+        ph = torch.empty(B, device="cuda")
+        e = torch.empty(B, device="cuda")
+        m.logpsi(r, Rd, Zd, lp, ph)
+        m.local_energy(r, Rd, Zd, e)
+        torch.cuda.synchronize()
+        return lp.cpu().numpy(), ph.cpu().numpy(), e.cpu().numpy()
+
+    m.set_tuning(fwd_main_tile=-1)
+    lp_rows, ph_rows, e_rows = run()
+    for tile in (4, 0, 2):
+        m.set_tuning(fwd_main_tile=tile)
+        lp, ph, e = run()
+        np.testing.assert_array_equal(lp, lp_rows)
+        np.testing.assert_array_equal(ph, ph_rows)
+        np.testing.assert_array_equal(e, e_rows)
+    assert np.all(np.isfinite(lp_rows)) and np.all(np.isfinite(e_rows))
+
+
 def test_caller_workspace_gives_identical_results(cuda):
     """dw_model_set_workspace: the same passes with activation buffers carved from a caller buffer."""
     from deeperwin_amd._native import DwError, NativeModel
