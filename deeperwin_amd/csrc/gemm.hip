@@ -542,9 +542,10 @@ static void gemm_dispatch(GemmPrec prec, const float* A, int lda, const float* B
                           hipStream_t s) {
   if (prec == GEMM_F32) {
     // Bt holds pad128(N) rows, so every tile shape reads inside it.  Tile 4 (112 x 256 full-width rows,
-    // N = 256 only) is what the heuristic takes where it would pick 64 x 64 tiles.
+    // N = 256 only) is what the heuristic takes where it would pick 64 x 64 tiles and the row tiles fill
+    // at least 192 CUs (one workgroup per CU).
     int tile = ep.tile >= 0 ? ep.tile : pick_tile((long)M * ep.ksplit, N);
-    if (ep.tile < 0 && tile == 0) tile = 4;
+    if (ep.tile < 0 && tile == 0 && (M + 111) / 112 >= 192) tile = 4;
     if (tile == 4 && !(N == 256 && ep.ksplit == 1 && K % 16 == 0 && K >= 16 && (!ep.V || (ep.C == 1 && ep.n >= 8))))
       tile = 0;
     if (tile == 4) {

@@ -243,7 +243,8 @@ dw_status dw_model_set_chunk(dw_model* m, int32_t max_walkers_per_pass);
 /* Launch-shape choices of one model (no counterpart in the reference: XLA picks its own tiles).  Set by
  * the autotuner in deeperwin_amd/tuning.py per electron-count bucket; -1 / 0 keep the built-in heuristic.
  * FP32 GEMM tiles: 0 = 64x64, 1 = 64x128, 2 = 128x128 per 256-thread workgroup; fwd_main_tile also takes
- * 4 = 112 rows x 256 columns (N = 256; the heuristic's choice where it would take 64x64). */
+ * 4 = 112 rows x 256 columns (N = 256; the heuristic's choice where it would take 64x64 and that gives at
+ * least 192 row tiles). */
 typedef struct dw_tuning {
   int32_t fwd_main_tile;  /* 1-el embedding GEMMs of the value pass (3: 96 x 128 tiles, 16x16 MFMA)   */
   int32_t fwd_orb_tile;   /* orbital GEMMs of the value pass                                         */

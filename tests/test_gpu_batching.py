@@ -138,9 +138,9 @@ def test_tuning_changes_launch_shapes_only(cuda):
 
 
 def test_row_tile_value_gemm_matches_small_tiles(cuda):
-    """The full-width row-tile value GEMM (the heuristic's choice at 14 electrons: V sums staged per
-    workgroup, 112-row tiles, here with a ragged last tile: 500 walkers = 7000 rows) sums the same
-    products in the same order as the 64 x 64 tiles: log|psi|, phase and E_loc are bit-equal."""
+    """The full-width row-tile value GEMM (tile 4: V sums staged per workgroup, 112-row tiles, here with
+    a ragged last tile: 500 walkers = 7000 rows) sums the same products in the same order as the 64 x 64
+    and 128 x 128 tiles: log|psi|, phase and E_loc are bit-equal."""
     from deeperwin_amd._native import NativeModel
     from deeperwin_amd.params import init_params
     from deeperwin_amd.physics import generate_el_ion_mapping, geometry_n2, init_walkers_gaussian
@@ -165,9 +165,9 @@ def test_row_tile_value_gemm_matches_small_tiles(cuda):
         torch.cuda.synchronize()
         return lp.cpu().numpy(), ph.cpu().numpy(), e.cpu().numpy()
 
-    m.set_tuning(fwd_main_tile=-1)
+    m.set_tuning(fwd_main_tile=4)
     lp_rows, ph_rows, e_rows = run()
-    for tile in (4, 0, 2):
+    for tile in (-1, 0, 2):
         m.set_tuning(fwd_main_tile=tile)
         lp, ph, e = run()
         np.testing.assert_array_equal(lp, lp_rows)
