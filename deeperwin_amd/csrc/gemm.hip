@@ -388,21 +388,22 @@ static void launch_f32(const float* A, int lda, const float* Bt, int ldb, float*
 }
 
 // ------------------------------------------------------------------------------------------------
-// FP32 MFMA on full-width row tiles: 112 rows x all 256 columns per 1024-thread workgroup (the A tile is
-// read once; at 28,672 rows, one workgroup per CU), wave w owning columns [16 w, 16 w + 16) of all rows
-// (7 blocks of 16 x 16; four waves per SIMD hide each other's LDS and barrier waits: measured faster than
-// 8 waves of 7 x 2 blocks and 4 waves of 7 x 4 blocks).  Both operand slices are
+// FP32 MFMA on full-width row tiles: 16 RB rows (112 for the value pass, 128 for the large orbital GEMMs)
+// x all columns (N <= 256) per 1024-thread workgroup (the A tile is read once; at 28,672 rows and RB = 7,
+// one workgroup per CU), wave w owning columns [16 w, 16 w + 16) of all rows (RB blocks of 16 x 16; four
+// waves per SIMD hide each other's LDS and barrier waits: measured faster than 8 waves of 7 x 2 blocks and
+// 4 waves of 7 x 4 blocks); waves whose columns lie past N only stage operands.  Both operand slices are
 // double-buffered in LDS (one barrier per 16-wide slice) with the next slice's loads in registers.  The
 // slices, the LDS layout and the MFMA step order (step s of lane group g takes k = 4 g + s) are
-// k_gemm_f32's, and so is the epilogue: results are bit-identical to it.  N = 256, K % 16 == 0, no split-K;
-// with a V epilogue, C == 1 and n >= 8.
+// k_gemm_f32's, and so is the epilogue: results are bit-identical to it.  128 < N <= 256, N % 16 == 0 (Bt
+// then holds 256 rows), K % 16 == 0, no split-K; with a V epilogue, RB = 7, N = 256, C == 1 and n >= 8.
 // ------------------------------------------------------------------------------------------------
-template <int FLAGS>
+template <int FLAGS, int RB>
 __global__ __launch_bounds__(1024) void k_gemm_rows256(const float* __restrict__ A, int lda,
                                                       const float* __restrict__ Bt, int ldb, float* __restrict__ Cm,
-                                                      int ldc, int M, int K, int tilesTotal, GemmEpi ep) {
+                                                      int ldc, int M, int N, int K, int tilesTotal, GemmEpi ep) {
   constexpr int NT = 1024;  // 16 waves, 4 per SIMD
-  constexpr int RB = 7, BM = 16 * RB, BN = 256, BK = 16, PITCH = BK + 4, KV = BK / 4, NQ = 1;
+  constexpr int BM = 16 * RB, BN = 256, BK = 16, PITCH = BK + 4, KV = BK / 4, NQ = 1;
   constexpr int AV = BM * KV, BV = BN * KV, AIT = (AV + NT - 1) / NT, BIT = BV / NT;
   __shared__ __align__(16) float As[2][BM][PITCH];
   __shared__ __align__(16) float Bs[2][BN][PITCH];
@@ -451,24 +452,27 @@ __global__ __launch_bounds__(1024) void k_gemm_rows256(const float* __restrict__
   };
 
   const int S = K / BK;
+  const bool active = wid * 16 < N;  // uniform over the wave
   gload(0);
   sstore(0);
   if (S > 1) gload(BK);
   __syncthreads();
   for (int sl = 0; sl < S; ++sl) {
     const int buf = sl & 1;
-    v4f a4[RB], b4[NQ];
+    if (active) {
+      v4f a4[RB], b4[NQ];
 #pragma unroll
-    for (int i = 0; i < RB; ++i) a4[i] = *(const v4f*)&As[buf][i * 16 + cl][4 * g];
+      for (int i = 0; i < RB; ++i) a4[i] = *(const v4f*)&As[buf][i * 16 + cl][4 * g];
 #pragma unroll
-    for (int j = 0; j < NQ; ++j) b4[j] = *(const v4f*)&Bs[buf][wid * 16 * NQ + j * 16 + cl][4 * g];
+      for (int j = 0; j < NQ; ++j) b4[j] = *(const v4f*)&Bs[buf][wid * 16 * NQ + j * 16 + cl][4 * g];
 #pragma unroll
-    for (int st = 0; st < 4; ++st)
+      for (int st = 0; st < 4; ++st)
 #pragma unroll
-      for (int i = 0; i < RB; ++i)
+        for (int i = 0; i < RB; ++i)
 #pragma unroll
-        for (int j = 0; j < NQ; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[i][st], b4[j][st], acc[i][j], 0, 0, 0);
+          for (int j = 0; j < NQ; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[i][st], b4[j][st], acc[i][j], 0, 0, 0);
+    }
     // the other buffer was last read before the previous barrier; the next slice's compute reads it
     // after this one
     if (sl + 1 < S) {
@@ -482,7 +486,7 @@ __global__ __launch_bounds__(1024) void k_gemm_rows256(const float* __restrict__
   // 15 walkers) the walkers' V sums (parts added in epi_store's order) are formed once in LDS after the
   // main loop's last barrier, and the 112 outputs of a lane read no V from memory.
   constexpr int WMAX = 16;
-  static_assert(WMAX * BN <= 2 * BM * PITCH, "V staging fits in the A buffers");
+  static_assert((FLAGS & EPI_V) == 0 || WMAX * BN <= 2 * BM * PITCH, "V staging fits in the A buffers");
   float(*Vs)[BN] = (float(*)[BN]) & As[0][0][0];
   int b_first = 0;
   if constexpr ((FLAGS & EPI_V) != 0) {
@@ -506,7 +510,7 @@ __global__ __launch_bounds__(1024) void k_gemm_rows256(const float* __restrict__
       for (int e = 0; e < 4; ++e) {
         const int r = (int)row0 + i * 16 + 4 * g + e;
         const int c = wid * 16 * NQ + j * 16 + cl;
-        if (r < M) {
+        if (r < M && c < N) {
           if constexpr ((FLAGS & EPI_V) != 0) {
             // epi_store with C == 1: bias on every row, then V, tanh, spin-major permutation
             float v = acc[i][j][e];
@@ -546,14 +550,26 @@ static void gemm_dispatch(GemmPrec prec, const float* A, int lda, const float* B
     // at least 192 CUs (one workgroup per CU).
     int tile = ep.tile >= 0 ? ep.tile : pick_tile((long)M * ep.ksplit, N);
     if (ep.tile < 0 && tile == 0 && (M + 111) / 112 >= 192) tile = 4;
+    // Plain GEMMs with 128 < N <= 256 and at least 1024 row tiles of 128 (the Laplacian-pass orbital GEMMs:
+    // 688,128 x 256 x 224 at N2, 888 -> ~790 us each): full-width 128-row tiles, waves past N idle in the
+    // MFMA phase.
+    if constexpr (FLAGS == 0) {
+      if (ep.tile < 0 && N > 128 && N <= 256 && N % 16 == 0 && ep.ksplit == 1 && K % 16 == 0 && K >= 16 &&
+          (M + 127) / 128 >= 1024) {
+        const int total = (M + 127) / 128;
+        hipLaunchKernelGGL((k_gemm_rows256<FLAGS, 8>), dim3((total + 7) / 8 * 8), dim3(1024), 0, s, A, lda, Bt, ldb,
+                           C, ldc, M, N, K, total, ep);
+        return;
+      }
+    }
     if (tile == 4 && !(N == 256 && ep.ksplit == 1 && K % 16 == 0 && K >= 16 && (!ep.V || (ep.C == 1 && ep.n >= 8))))
       tile = 0;
     if (tile == 4) {
       // (not for the generic all-flags instance, whose epilogue would not stay in registers)
       if constexpr (FLAGS != (EPI_BIAS | EPI_V | EPI_TANH | EPI_PERM)) {
         const int total = (M + 111) / 112;
-        hipLaunchKernelGGL((k_gemm_rows256<FLAGS>), dim3((total + 7) / 8 * 8), dim3(1024), 0, s, A, lda, Bt, ldb, C,
-                           ldc, M, K, total, ep);
+        hipLaunchKernelGGL((k_gemm_rows256<FLAGS, 7>), dim3((total + 7) / 8 * 8), dim3(1024), 0, s, A, lda, Bt, ldb,
+                           C, ldc, M, N, K, total, ep);
         return;
       }
       tile = 0;

@@ -138,15 +138,16 @@ def test_tuning_changes_launch_shapes_only(cuda):
 
 
 def test_row_tile_value_gemm_matches_small_tiles(cuda):
-    """The full-width row-tile value GEMM (tile 4: V sums staged per workgroup, 112-row tiles, here with
-    a ragged last tile: 500 walkers = 7000 rows) sums the same products in the same order as the 64 x 64
-    and 128 x 128 tiles: log|psi|, phase and E_loc are bit-equal."""
+    """The full-width row-tile GEMMs sum the same products in the same order as the 64 x 64 and 128 x 128
+    tiles: the value GEMM (tile 4: V sums staged per workgroup, 112-row tiles, here with a ragged last
+    tile: 500 walkers = 7000 rows) and the Laplacian-pass orbital GEMMs (the heuristic's 128-row tiles
+    at 168,000 rows x 224 orbitals).  log|psi|, phase and E_loc are bit-equal."""
     from deeperwin_amd._native import NativeModel
     from deeperwin_amd.params import init_params
     from deeperwin_amd.physics import generate_el_ion_mapping, geometry_n2, init_walkers_gaussian
 
     R, Z = geometry_n2()
-    shape = ModelShape(n_el=14, n_up=7, n_ion=2, n_det=4)
+    shape = ModelShape(n_el=14, n_up=7, n_ion=2, n_det=16)
     params = init_params(shape, 3)
     B = 500
     r0 = init_walkers_gaussian(np.random.default_rng(4), B, R, generate_el_ion_mapping(R, Z, 14, 7))
@@ -165,10 +166,10 @@ def test_row_tile_value_gemm_matches_small_tiles(cuda):
         torch.cuda.synchronize()
         return lp.cpu().numpy(), ph.cpu().numpy(), e.cpu().numpy()
 
-    m.set_tuning(fwd_main_tile=4)
+    m.set_tuning(fwd_main_tile=4, lap_orb_tile=-1)
     lp_rows, ph_rows, e_rows = run()
     for tile in (-1, 0, 2):
-        m.set_tuning(fwd_main_tile=tile)
+        m.set_tuning(fwd_main_tile=tile, lap_orb_tile=tile)
         lp, ph, e = run()
         np.testing.assert_array_equal(lp, lp_rows)
         np.testing.assert_array_equal(ph, ph_rows)
