@@ -551,7 +551,7 @@ static void gemm_dispatch(GemmPrec prec, const float* A, int lda, const float* B
     int tile = ep.tile >= 0 ? ep.tile : pick_tile((long)M * ep.ksplit, N);
     if (ep.tile < 0 && tile == 0 && (M + 111) / 112 >= 192) tile = 4;
     // Plain GEMMs with 128 < N <= 256 and at least 1024 row tiles of 128 (the Laplacian-pass orbital GEMMs:
-    // 688,128 x 256 x 224 at N2, 888 -> ~790 us each): full-width 128-row tiles, waves past N idle in the
+    // 688,128 x 256 x 224 at N2, 888 -> 816 us each): full-width 128-row tiles, waves past N idle in the
     // MFMA phase.
     if constexpr (FLAGS == 0) {
       if (ep.tile < 0 && N > 128 && N <= 256 && N % 16 == 0 && ep.ksplit == 1 && K % 16 == 0 && K >= 16 &&
